@@ -1,0 +1,240 @@
+"""Pythonic wrappers over the C ABI (configuration, streams, sinks)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import (CODEC_ID, DTYPE_ID, NP_DTYPE, REDUCE_ID, CodecConfig, Dimension, MemoryInfo,
+               Slice, StreamOptions, TileStreamConfiguration, lib)
+
+
+def last_error() -> str:
+    v = lib.chucky_last_error()
+    return v.decode() if v else ""
+
+
+def crc32c(data: bytes) -> int:
+    buf = C.create_string_buffer(bytes(data), len(data))
+    return lib.chucky_crc32c(C.cast(buf, C.c_void_p), len(data))
+
+
+@dataclass
+class Dim:
+    size: int
+    chunk: int
+    cps: int = 0
+    name: str = ""
+    downsample: bool = False
+    storage_position: Optional[int] = None
+
+
+@dataclass
+class Config:
+    """Mirror of struct tile_stream_configuration (include/types.stream.h)."""
+    dims: Sequence[Dim]
+    dtype: str = "u16"
+    codec: str = "none"
+    level: int = 1
+    reduce: str = "mean"
+    append_reduce: str = "mean"
+    max_nlod: int = 0
+    epochs_per_batch: int = 0
+    target_batch_chunks: int = 0
+    buffer_capacity_bytes: int = 1 << 20
+    shard_alignment: int = 0
+    metadata_update_interval_s: float = 1e9
+    _keep: list = field(default_factory=list, repr=False)
+
+    def to_c(self) -> TileStreamConfiguration:
+        rank = len(self.dims)
+        arr = (Dimension * rank)()
+        names = []
+        for i, d in enumerate(self.dims):
+            nm = (d.name or "abcdefghijklmnopqrstuvwxyz"[i]).encode()
+            names.append(nm)
+            arr[i].size = d.size
+            arr[i].chunk_size = d.chunk
+            arr[i].chunks_per_shard = d.cps
+            arr[i].name = nm
+            arr[i].downsample = 1 if d.downsample else 0
+            arr[i].storage_position = i if d.storage_position is None else d.storage_position
+            arr[i].ngff.scale = 1.0
+        cfg = TileStreamConfiguration()
+        cfg.buffer_capacity_bytes = self.buffer_capacity_bytes
+        cfg.dtype = DTYPE_ID[self.dtype]
+        cfg.rank = rank
+        cfg.dimensions = C.cast(arr, C.POINTER(Dimension))
+        cfg.codec = CodecConfig(CODEC_ID[self.codec], self.level, 0)
+        cfg.reduce_method = REDUCE_ID[self.reduce]
+        cfg.append_reduce_method = REDUCE_ID[self.append_reduce]
+        cfg.max_nlod = self.max_nlod
+        cfg.epochs_per_batch = self.epochs_per_batch
+        cfg.target_batch_chunks = self.target_batch_chunks
+        cfg.metadata_update_interval_s = self.metadata_update_interval_s
+        cfg.shard_alignment = self.shard_alignment
+        cfg.max_threads = 0
+        self._keep[:] = [arr, names, cfg]
+        return cfg
+
+    @property
+    def np_dtype(self):
+        return np.dtype(NP_DTYPE[self.dtype])
+
+
+class MemSink:
+    """In-memory shard sink (chucky_mem_sink_*)."""
+
+    def __init__(self):
+        self.h = lib.chucky_mem_sink_create()
+
+    @property
+    def shard_sink(self):
+        return lib.chucky_mem_sink_as_shard_sink(self.h)
+
+    def shards(self) -> Dict[Tuple[int, int], bytes]:
+        out = {}
+        n = lib.chucky_mem_sink_count(self.h)
+        for i in range(n):
+            lv = C.c_uint8()
+            si = C.c_uint64()
+            data = C.POINTER(C.c_uint8)()
+            size = C.c_size_t()
+            fin = C.c_int()
+            lib.chucky_mem_sink_get(self.h, i, C.byref(lv), C.byref(si), C.byref(data),
+                                    C.byref(size), C.byref(fin))
+            out[(lv.value, si.value)] = C.string_at(data, size.value) if size.value else b""
+        return out
+
+    def finalized(self) -> Dict[Tuple[int, int], bool]:
+        out = {}
+        for i in range(lib.chucky_mem_sink_count(self.h)):
+            lv = C.c_uint8()
+            si = C.c_uint64()
+            fin = C.c_int()
+            lib.chucky_mem_sink_get(self.h, i, C.byref(lv), C.byref(si), None, None, C.byref(fin))
+            out[(lv.value, si.value)] = bool(fin.value)
+        return out
+
+    def append_sizes(self, level: int) -> Optional[List[int]]:
+        buf = (C.c_uint64 * 64)()
+        n = C.c_uint8()
+        if lib.chucky_mem_sink_append_sizes(self.h, level, buf, C.byref(n)):
+            return None
+        return list(buf[: n.value])
+
+    def inject_failure(self, fail_open=False, fail_write=False):
+        lib.chucky_mem_sink_inject_failure(self.h, int(fail_open), int(fail_write))
+
+    def close(self):
+        if self.h:
+            lib.chucky_mem_sink_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DiscardSink:
+    def __init__(self):
+        self.h = lib.chucky_discard_sink_create()
+
+    @property
+    def shard_sink(self):
+        return lib.chucky_discard_sink_as_shard_sink(self.h)
+
+    @property
+    def bytes(self) -> int:
+        return lib.chucky_discard_sink_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib.chucky_discard_sink_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def memory_estimate(cfg: Config) -> MemoryInfo:
+    info = MemoryInfo()
+    c = cfg.to_c()
+    if lib.tile_stream_gpu_memory_estimate(C.byref(c), C.byref(info)):
+        raise ValueError("invalid configuration")
+    return info
+
+
+class GpuStream:
+    """tile_stream_gpu_* through its writer vtable."""
+
+    def __init__(self, cfg: Config, sink, pipeline: int = 0, workers: int = 0):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        self.sink = sink
+        opts = StreamOptions()
+        lib.tile_stream_gpu_options_default(C.byref(opts))
+        opts.pipeline = pipeline
+        opts.compress_workers = workers
+        self.h = lib.tile_stream_gpu_create_ex(C.byref(self._c), sink.shard_sink, C.byref(opts))
+        if not self.h:
+            raise RuntimeError("tile_stream_gpu_create failed: " + last_error())
+        self.w = lib.tile_stream_gpu_writer(self.h)
+
+    @property
+    def layout(self):
+        return lib.tile_stream_gpu_layout(self.h).contents
+
+    @property
+    def epoch_elements(self) -> int:
+        return self.layout.epoch_elements
+
+    @property
+    def cursor(self) -> int:
+        return lib.tile_stream_gpu_cursor(self.h)
+
+    def status(self):
+        return lib.tile_stream_gpu_status(self.h)
+
+    def level_geometry(self, level: int):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        if lib.tile_stream_gpu_level_geometry(self.h, level, C.byref(a), C.byref(b), C.byref(c)):
+            raise ValueError("bad level")
+        return a.value, b.value, c.value
+
+    def metrics(self):
+        return lib.tile_stream_gpu_get_metrics(self.h)
+
+    def append(self, data) -> int:
+        """Append host bytes (numpy array or bytes). Returns writer error code."""
+        buf = np.ascontiguousarray(data)
+        beg = buf.ctypes.data
+        r = lib.writer_append(self.w, Slice(beg, beg + buf.nbytes))
+        self._last_rest = (r.rest.beg or 0) - beg if r.rest.beg else 0
+        return r.error
+
+    def append_device(self, ptr: int, nbytes: int) -> int:
+        r = lib.tile_stream_gpu_append_device(self.h, ptr, nbytes)
+        return r.error
+
+    def flush(self) -> int:
+        return lib.writer_flush(self.w).error
+
+    def reset(self, sink=None) -> int:
+        if sink is not None:
+            self.sink = sink
+        return lib.tile_stream_gpu_reset(self.h, sink.shard_sink if sink is not None else None)
+
+    def latencies(self) -> List[float]:
+        buf = (C.c_double * 65536)()
+        n = lib.tile_stream_gpu_epoch_latencies(self.h, buf, 65536)
+        return list(buf[:n])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.tile_stream_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,165 @@
+// Shard aggregation on the device.
+//
+// Reference semantics (src/cpu/aggregate.c:129-188, src/gpu/aggregate.cu:357-438):
+//   permuted[perm[i]] = size(gather[i]); chunk_sizes = permuted (unpadded);
+//   the last slot of every group of n_active*cps_inner slots is padded so the
+//   group ends on a page boundary; offsets = exclusive scan of the padded sizes;
+//   bytes of item i are copied to data + offsets[perm[i]].
+//
+// gfx950 design: the size permutation, padding and scan are one single-
+// workgroup kernel (slot counts are thousands, so one CU finishes it in a few
+// microseconds and no host round trip or device-wide scan is needed). The byte
+// gather runs one workgroup per (item, segment) and writes aligned dwords:
+// each lane funnel-shifts two source dwords into one destination dword, so
+// unaligned destination offsets still produce full-width stores.
+#include "kernels.hpp"
+
+namespace chucky {
+namespace {
+
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ uint64_t item_bytes(const SegmentSource& s, uint32_t chunk)
+{
+  if (!s.seg_sizes)
+    return s.const_total;
+  uint64_t t = 0;
+  for (uint32_t k = 0; k < s.nseg; ++k)
+    t += s.seg_sizes[(uint64_t)chunk * s.nseg + k];
+  return t;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
+  SegmentSource src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
+  uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* chunk_sizes,
+  uint64_t* offsets)
+{
+  __shared__ uint64_t part[kPlanThreads];
+  const int t = threadIdx.x;
+  for (uint64_t i = t; i < n_slots; i += kPlanThreads)
+    chunk_sizes[i] = 0;
+  __syncthreads();
+  for (uint64_t i = t; i < n_items; i += kPlanThreads)
+    chunk_sizes[perm[i]] = item_bytes(src, gather[i]);
+  __syncthreads();
+
+  // Contiguous range per thread; padding is added on the fly to the value of
+  // the last slot in each group (the unpadded size stays in chunk_sizes).
+  const uint64_t per = (n_slots + kPlanThreads - 1) / kPlanThreads;
+  const uint64_t b = (uint64_t)t * per;
+  const uint64_t e = b + per < n_slots ? b + per : n_slots;
+  auto value = [&](uint64_t i) -> uint64_t {
+    uint64_t v = chunk_sizes[i];
+    if (page && group && (i % group) == group - 1) {
+      uint64_t tot = 0;
+      for (uint64_t j = i + 1 - group; j <= i; ++j)
+        tot += chunk_sizes[j];
+      v += (tot + page - 1) / page * page - tot;
+    }
+    return v;
+  };
+  uint64_t local = 0;
+  for (uint64_t i = b; i < e; ++i)
+    local += value(i);
+  part[t] = local;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the 1024 partials.
+  for (int off = 1; off < kPlanThreads; off <<= 1) {
+    const uint64_t add = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  uint64_t run = part[t] - local;
+  for (uint64_t i = b; i < e; ++i) {
+    offsets[i] = run;
+    run += value(i);
+  }
+  if (t == kPlanThreads - 1)
+    offsets[n_slots] = part[t];
+}
+
+__device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p)
+{
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+  const uint32_t sh = (uint32_t)(a & 3) * 8;
+  if (sh == 0)
+    return w[0];
+  return (uint32_t)(((uint64_t)w[1] << 32 | w[0]) >> sh);
+}
+
+// grid.x = item, grid.y = segment (strided). Copies the segment's bytes.
+__global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
+                                                          const uint32_t* __restrict__ gather,
+                                                          const uint32_t* __restrict__ perm,
+                                                          uint64_t n_items,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          uint8_t* __restrict__ dst)
+{
+  const uint64_t item = blockIdx.x;
+  if (item >= n_items)
+    return;
+  const uint32_t chunk = gather[item];
+  const uint64_t base_out = offsets[perm[item]];
+  for (uint32_t seg = blockIdx.y; seg < src.nseg; seg += gridDim.y) {
+    uint64_t seg_off = 0, nbytes;
+    if (src.seg_sizes) {
+      for (uint32_t k = 0; k < seg; ++k)
+        seg_off += src.seg_sizes[(uint64_t)chunk * src.nseg + k];
+      nbytes = src.seg_sizes[(uint64_t)chunk * src.nseg + seg];
+    } else {
+      seg_off = (uint64_t)seg * src.const_size;
+      nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
+    }
+    if (nbytes == 0)
+      continue;
+    const uint8_t* in = src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride;
+    uint8_t* out = dst + base_out + seg_off;
+    // head bytes up to a dword boundary of the destination
+    const uint32_t head = (uint32_t)((4 - ((uintptr_t)out & 3)) & 3);
+    const uint64_t h = head < nbytes ? head : nbytes;
+    if (threadIdx.x < h)
+      out[threadIdx.x] = in[threadIdx.x];
+    const uint64_t body = (nbytes - h) / 4;
+    uint32_t* ow = (uint32_t*)(out + h);
+    const uint8_t* ib = in + h;
+    if (((uintptr_t)ib & 3) == 0) {
+      const uint32_t* iw = (const uint32_t*)ib;
+      for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
+        ow[w] = iw[w];
+    } else {
+      for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
+        ow[w] = load_u32_unaligned(ib + 4 * w);
+    }
+    const uint64_t tail0 = h + body * 4;
+    if (threadIdx.x < nbytes - tail0)
+      out[tail0 + threadIdx.x] = in[tail0 + threadIdx.x];
+  }
+}
+
+} // namespace
+
+hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gather,
+                                 const uint32_t* perm, uint64_t n_items, uint64_t n_slots,
+                                 uint64_t group, uint64_t page_size, uint64_t* chunk_sizes,
+                                 uint64_t* offsets, hipStream_t stream)
+{
+  hipLaunchKernelGGL(aggregate_plan_k, dim3(1), dim3(kPlanThreads), 0, stream, src, gather, perm,
+                     n_items, n_slots, group, page_size, chunk_sizes, offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gather,
+                                   const uint32_t* perm, uint64_t n_items, const uint64_t* offsets,
+                                   uint8_t* dst, hipStream_t stream)
+{
+  if (n_items == 0)
+    return hipSuccess;
+  const uint32_t gy = src.nseg < 64 ? src.nseg : 64;
+  hipLaunchKernelGGL(aggregate_gather_k, dim3((uint32_t)n_items, gy), dim3(256), 0, stream, src,
+                     gather, perm, n_items, offsets, dst);
+  return hipGetLastError();
+}
+
+} // namespace chucky

@@ -1,0 +1,116 @@
+// Kernel-level C entry points (device pointers, synchronous on the null
+// stream of the current device). Used by parity tests and the autotuner.
+#include "chucky_amd.h"
+#include "kernels.hpp"
+
+#include <string>
+
+namespace chucky {
+void set_error(const std::string& msg);
+}
+using namespace chucky;
+
+extern "C" {
+
+size_t chucky_gpu_max_output_size(enum compression_codec codec, size_t chunk_bytes)
+{
+  switch (codec) {
+    case CODEC_NONE: return chunk_bytes;
+    case CODEC_LZ4_NON_STANDARD: return lz4_max_output_size(chunk_bytes);
+    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes);
+    default: return 0;
+  }
+}
+
+int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_src,
+                        size_t src_stride, size_t chunk_bytes, uint64_t n, void* d_dst,
+                        size_t dst_stride, uint64_t* d_sizes)
+{
+  if (n == 0)
+    return 0;
+  if (dst_stride < chucky_gpu_max_output_size(codec, chunk_bytes)) {
+    set_error("chucky_gpu_compress: dst_stride below the codec bound");
+    return 1;
+  }
+  hipStream_t st = nullptr;
+  uint32_t nseg = codec == CODEC_ZSTD ? zstd_segments(chunk_bytes) : 1;
+  uint32_t* seg = nullptr;
+  void* scratch = nullptr;
+  uint32_t* queue = nullptr;
+  int rc = 1;
+  if (hipStreamCreate(&st) != hipSuccess)
+    return 1;
+  if (hipMalloc((void**)&seg, n * nseg * 4 + 256) != hipSuccess)
+    goto done;
+  if (codec == CODEC_NONE) {
+    set_error("chucky_gpu_compress: CODEC_NONE has no compressor");
+    goto done;
+  } else if (codec == CODEC_LZ4_NON_STANDARD) {
+    if (launch_lz4((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
+                   seg, 12, st) != hipSuccess)
+      goto done;
+  } else if (codec == CODEC_ZSTD) {
+    const uint32_t workers = (uint32_t)(n * nseg < 2048 ? n * nseg : 2048);
+    if (hipMalloc(&scratch, zstd_scratch_bytes(workers)) != hipSuccess ||
+        hipMalloc((void**)&queue, 256) != hipSuccess)
+      goto done;
+    if (launch_zstd((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
+                    seg, level, scratch, workers, queue, st) != hipSuccess)
+      goto done;
+  } else {
+    set_error("chucky_gpu_compress: unsupported codec");
+    goto done;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess)
+    goto done;
+  {
+    // Per-chunk compressed bytes. For zstd the segments of a chunk sit at a
+    // fixed stride inside its slot; compact them so each chunk's frame is
+    // contiguous at d_dst + i*dst_stride.
+    std::basic_string<uint32_t> h(n * nseg, 0);
+    if (hipMemcpy(&h[0], seg, n * nseg * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      goto done;
+    std::basic_string<uint64_t> sizes(n, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+      uint64_t off = h[i * nseg];
+      for (uint32_t k = 1; k < nseg; ++k) {
+        uint8_t* base = (uint8_t*)d_dst + i * dst_stride;
+        if (hipMemcpy(base + off, base + (uint64_t)k * zstd_segment_stride(), h[i * nseg + k],
+                      hipMemcpyDeviceToDevice) != hipSuccess)
+          goto done;
+        off += h[i * nseg + k];
+      }
+      sizes[i] = off;
+    }
+    if (hipMemcpy(d_sizes, sizes.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
+      goto done;
+  }
+  rc = 0;
+done:
+  if (rc && chucky_last_error()[0] == 0)
+    set_error("chucky_gpu_compress failed");
+  (void)hipFree(seg);
+  (void)hipFree(scratch);
+  (void)hipFree(queue);
+  (void)hipStreamDestroy(st);
+  return rc;
+}
+
+int chucky_gpu_transpose(void* d_dst, const void* d_src, uint64_t n, uint8_t bpe, uint64_t i_offset,
+                         uint8_t lifted_rank, const uint64_t* lifted_shape,
+                         const int64_t* lifted_strides)
+{
+  if (lifted_rank == 0 || lifted_rank > 64 || (bpe != 1 && bpe != 2 && bpe != 4 && bpe != 8))
+    return 1;
+  LiftedSpace sp{};
+  sp.rank = lifted_rank;
+  for (int d = 0; d < lifted_rank; ++d) {
+    sp.shape[d] = lifted_shape[d];
+    sp.strides[d] = lifted_strides[d];
+  }
+  if (launch_transpose(d_dst, d_src, n, bpe, i_offset, sp, nullptr) != hipSuccess)
+    return 1;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 1;
+}
+
+} // extern "C"

@@ -1,0 +1,76 @@
+// Launch wrappers of the gfx950 kernels. All launches are asynchronous on the
+// given stream and return hipSuccess or the launch error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace chucky {
+
+// ---- transpose (scatter into chunk-major pool) ----
+
+struct LiftedSpace
+{
+  int rank;                // lifted rank (2 * array rank)
+  uint64_t shape[64];
+  int64_t strides[64];
+};
+
+// dst[ravel(space, i_offset + i)] = src[i] for i < n. i_offset may be any
+// element position of the stream; only its value modulo the epoch matters.
+hipError_t launch_transpose(void* dst, const void* src, uint64_t n, int bpe, uint64_t i_offset,
+                            const LiftedSpace& space, hipStream_t stream);
+
+// ---- aggregate ----
+
+// Sizes of the compressed segments. A chunk is nseg segments; segment s of
+// chunk c lives at base + c*chunk_stride + s*seg_stride and holds
+// seg_sizes[c*nseg+s] bytes (or const_size bytes when seg_sizes is null).
+struct SegmentSource
+{
+  const uint8_t* base;
+  uint64_t chunk_stride;
+  uint64_t seg_stride;
+  uint32_t nseg;
+  const uint32_t* seg_sizes;
+  uint64_t const_size;      // used when seg_sizes == nullptr (last segment gets the remainder)
+  uint64_t const_total;     // chunk bytes when seg_sizes == nullptr
+};
+
+// One workgroup: chunk_sizes[perm[i]] = size(gather[i]) for i < n_items,
+// zero elsewhere; optional page padding on the last slot of each group of
+// `group` slots; offsets = exclusive scan (n_slots + 1 entries).
+hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gather,
+                                 const uint32_t* perm, uint64_t n_items, uint64_t n_slots,
+                                 uint64_t group, uint64_t page_size, uint64_t* chunk_sizes,
+                                 uint64_t* offsets, hipStream_t stream);
+
+// Copy every item's segments to dst + offsets[perm[i]] (dst may be device or
+// pinned host memory).
+hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gather,
+                                   const uint32_t* perm, uint64_t n_items, const uint64_t* offsets,
+                                   uint8_t* dst, hipStream_t stream);
+
+// ---- codecs ----
+
+size_t lz4_max_output_size(size_t chunk_bytes);
+size_t zstd_max_output_size(size_t chunk_bytes);
+uint32_t zstd_segments(size_t chunk_bytes);
+uint64_t zstd_segment_stride();
+
+// LZ4 raw block per chunk: one wavefront per chunk.
+hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                      uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
+                      hipStream_t stream);
+
+// zstd frame per chunk, split into independent 128 KiB blocks (segments).
+// scratch: see zstd_scratch_bytes(workers).
+size_t zstd_scratch_bytes(uint32_t workers);
+hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                       uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
+                       void* scratch, uint32_t workers, uint32_t* queue_counter,
+                       hipStream_t stream);
+
+} // namespace chucky

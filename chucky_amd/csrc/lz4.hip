@@ -1,0 +1,186 @@
+// LZ4 raw-block compressor: one wavefront per chunk.
+//
+// Output format: one LZ4 block (no frame) per chunk, the same container the
+// reference writes with LZ4_compress_HC (src/cpu/compress.c:55-76); any
+// conforming decoder (LZ4_decompress_safe) restores the chunk. The bytes
+// themselves differ from LZ4-HC, which is allowed: chunks are checked by
+// round trip.
+//
+// Parse: greedy, 64 candidate positions per step. Every lane hashes the four
+// bytes at its position into an LDS table (4-byte keys, multiplicative hash),
+// verifies its candidate, and the wave takes the first lane with a match
+// (ballot + ffs). The match is extended 256 bytes per step with a ballot over
+// per-lane dword compares, extended backwards over pending literals, and
+// emitted: lane 0 writes token / length bytes / offset, the wave copies the
+// literal run with dword stores. Block-end rules of the LZ4 format are kept:
+// matches start at most 12 bytes before the end and end at least 5 bytes
+// before it; offsets are at most 65535.
+#include "kernels.hpp"
+#include "wave.hpp"
+
+namespace chucky {
+namespace {
+
+constexpr uint32_t kMaxOffset = 65535;
+constexpr uint32_t kMfLimit = 12;
+constexpr uint32_t kLastLiterals = 5;
+
+__device__ __forceinline__ uint32_t lz4_hash(uint32_t v, int hash_log)
+{
+  return (v * 2654435761u) >> (32 - hash_log);
+}
+
+__device__ __forceinline__ uint32_t put_len_ext(uint8_t* out, uint32_t op, uint32_t v)
+{
+  // v >= 15 was flagged in the token; write (v - 15) as 255-runs.
+  v -= 15;
+  while (v >= 255) {
+    out[op++] = 255;
+    v -= 255;
+  }
+  out[op++] = (uint8_t)v;
+  return op;
+}
+
+// Emits one sequence (literals [anchor, anchor+lit) then optionally a match).
+// Uniform arguments; returns the new output cursor.
+__device__ __forceinline__ uint32_t emit(uint8_t* out, uint32_t op, const uint8_t* in,
+                                         uint32_t anchor, uint32_t lit, bool has_match,
+                                         uint32_t mlen, uint32_t offset)
+{
+  const int lane = lane_id();
+  const uint32_t ml4 = has_match ? mlen - 4 : 0;
+  // header length is uniform; lane 0 writes it
+  uint32_t hdr = 1 + (lit >= 15 ? 1 + (lit - 15) / 255 : 0);
+  if (lane == 0) {
+    const uint8_t tok = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | (ml4 >= 15 ? 15 : ml4));
+    out[op] = tok;
+    if (lit >= 15)
+      put_len_ext(out, op + 1, lit);
+  }
+  op += hdr;
+  wave_copy(out + op, in + anchor, lit);
+  op += lit;
+  if (has_match) {
+    if (lane == 0) {
+      out[op] = (uint8_t)offset;
+      out[op + 1] = (uint8_t)(offset >> 8);
+      if (ml4 >= 15)
+        put_len_ext(out, op + 2, ml4);
+    }
+    op += 2 + (ml4 >= 15 ? 1 + (ml4 - 15) / 255 : 0);
+  }
+  return op;
+}
+
+template<int HASH_LOG>
+__global__ __launch_bounds__(64) void lz4_chunk_k(const uint8_t* __restrict__ src,
+                                                  uint64_t src_stride, uint32_t n, uint64_t nchunks,
+                                                  uint8_t* __restrict__ dst, uint64_t dst_stride,
+                                                  uint32_t* __restrict__ sizes)
+{
+  __shared__ uint32_t table[1 << HASH_LOG]; // position + 1, 0 = empty
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= nchunks)
+    return;
+  const int lane = lane_id();
+  const uint8_t* in = src + chunk * src_stride;
+  uint8_t* out = dst + chunk * dst_stride;
+  for (int i = lane; i < (1 << HASH_LOG); i += kWave)
+    table[i] = 0;
+  __syncthreads();
+
+  uint32_t op = 0, anchor = 0;
+  if (n > kMfLimit) {
+    const uint32_t last_start = n - kMfLimit;   // matches start at <= this
+    const uint32_t match_end = n - kLastLiterals; // and end at <= this
+    uint32_t pos = 0;
+    while (pos <= last_start) {
+      const uint32_t p = pos + (uint32_t)lane;
+      const bool active = p <= last_start;
+      uint32_t v = 0, h = 0, cand = 0;
+      if (active) {
+        v = ld32u(in + p);
+        h = lz4_hash(v, HASH_LOG);
+        cand = table[h];
+      }
+      // Lanes past the previous match may have left entries at or after the
+      // current positions: only strictly earlier candidates are matches.
+      bool hit = false;
+      if (active && cand != 0) {
+        const uint32_t c = cand - 1;
+        hit = c < p && (p - c) <= kMaxOffset && ld32u(in + c) == v;
+      }
+      // all lanes read before any lane writes (wave executes in lockstep)
+      __builtin_amdgcn_wave_barrier();
+      if (active)
+        table[h] = p + 1;
+      __builtin_amdgcn_wave_barrier();
+      const uint64_t mask = wave_ballot(hit);
+      if (mask == 0) {
+        pos += kWave;
+        continue;
+      }
+      const int first = __ffsll((unsigned long long)mask) - 1;
+      uint32_t mp = pos + (uint32_t)first;
+      uint32_t mc = wave_bcast(cand, first) - 1;
+      uint32_t len = wave_match_extend(in, mp, mc, 4, match_end);
+      // backward extension over pending literals
+      while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
+        --mp;
+        --mc;
+        ++len;
+      }
+      op = emit(out, op, in, anchor, mp - anchor, true, len, mp - mc);
+      pos = mp + len;
+      anchor = pos;
+      if (lane == 0 && pos - 2 <= last_start) {
+        const uint32_t q = pos - 2;
+        table[lz4_hash(ld32u(in + q), HASH_LOG)] = q + 1;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  op = emit(out, op, in, anchor, n - anchor, false, 0, 0);
+  if (lane == 0)
+    sizes[chunk] = op;
+}
+
+} // namespace
+
+size_t lz4_max_output_size(size_t chunk_bytes)
+{
+  // LZ4_COMPRESSBOUND: n + n/255 + 16
+  return chunk_bytes + chunk_bytes / 255 + 16;
+}
+
+hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                      uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
+                      hipStream_t stream)
+{
+  if (n == 0)
+    return hipSuccess;
+  if (chunk_bytes > 0x7E000000ull)
+    return hipErrorInvalidValue; // LZ4 block size limit
+  switch (hash_log) {
+    case 10:
+      hipLaunchKernelGGL(lz4_chunk_k<10>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
+                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
+      break;
+    case 11:
+      hipLaunchKernelGGL(lz4_chunk_k<11>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
+                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
+      break;
+    case 13:
+      hipLaunchKernelGGL(lz4_chunk_k<13>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
+                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
+      break;
+    default:
+      hipLaunchKernelGGL(lz4_chunk_k<12>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
+                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
+      break;
+  }
+  return hipGetLastError();
+}
+
+} // namespace chucky

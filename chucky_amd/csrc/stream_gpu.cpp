@@ -1,0 +1,878 @@
+// tile_stream_gpu_*: the drop-in streaming writer.
+//
+// Control flow mirrors the reference's stream (src/gpu/stream.c:65-245 for
+// the GPU, src/cpu/stream.c:527-762 for the CPU path whose output is the
+// parity target): appends are clipped at epoch boundaries and at the bounded
+// capacity; each full epoch is scattered (or LOD-reduced) into its slot of a
+// K-epoch chunk pool; every K epochs the batch is compressed, aggregated per
+// level and delivered to the shard sink; flush handles the partial epoch, the
+// partial batch, the append-dim drain and the final shard indices.
+//
+// Device work of the plain pipeline runs on one compute queue with a host
+// wait after the size scan (as the reference does at
+// src/gpu/flush.d2h_deliver.c:103-147); host-memory appends are staged through
+// a pinned double buffer on a separate copy queue.
+#include "stream_gpu.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+using namespace chucky;
+
+namespace {
+
+#define HIPCHECK(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+size_t gpu_max_output(enum compression_codec c, size_t chunk_bytes)
+{
+  switch (c) {
+    case CODEC_NONE: return chunk_bytes;
+    case CODEC_LZ4_NON_STANDARD: return lz4_max_output_size(chunk_bytes);
+    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes);
+    default: return 0;
+  }
+}
+
+constexpr size_t kSlack = 256; // every device buffer carries read slack for the codecs
+
+int dalloc(tile_stream_gpu* s, void** p, size_t bytes)
+{
+  *p = nullptr;
+  if (bytes == 0)
+    return 0;
+  HIPCHECK(hipMalloc(p, bytes + kSlack));
+  s->allocations.push_back(*p);
+  return 0;
+}
+
+int halloc(tile_stream_gpu* s, void** p, size_t bytes)
+{
+  *p = nullptr;
+  if (bytes == 0)
+    return 0;
+  HIPCHECK(hipHostMalloc(p, bytes + kSlack, hipHostMallocDefault));
+  s->host_allocations.push_back(*p);
+  return 0;
+}
+
+stream_metric mk_metric(const char* name)
+{
+  stream_metric m{};
+  m.name = name;
+  m.best_ms = 1e30f;
+  return m;
+}
+
+void add_metric(stream_metric& m, float ms, double in_bytes, double out_bytes)
+{
+  m.ms += ms;
+  m.input_bytes += in_bytes;
+  m.output_bytes += out_bytes;
+  m.count++;
+  if (ms < m.best_ms)
+    m.best_ms = ms;
+}
+
+float event_ms(hipEvent_t a, hipEvent_t b)
+{
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess)
+    return 0;
+  return ms;
+}
+
+LiftedSpace space_of(const tile_stream_layout& L)
+{
+  LiftedSpace sp{};
+  sp.rank = L.lifted_rank;
+  for (int d = 0; d < L.lifted_rank; ++d) {
+    sp.shape[d] = L.lifted_shape[d];
+    sp.strides[d] = L.lifted_strides[d];
+  }
+  return sp;
+}
+
+uint8_t* pool_epoch(tile_stream_gpu* s, uint32_t e) { return s->d_pool + (uint64_t)e * s->epoch_pool_bytes; }
+
+uint64_t epoch_elements(const tile_stream_gpu* s) { return s->plan.levels[0].layout.epoch_elements; }
+
+SegmentSource segment_source(const tile_stream_gpu* s)
+{
+  SegmentSource src{};
+  if (s->plan.config.codec.id == CODEC_NONE) {
+    src.base = s->d_pool;
+    src.chunk_stride = s->chunk_stride_bytes;
+    const uint64_t piece = 256 * 1024;
+    src.nseg = (uint32_t)std::max<uint64_t>(1, (s->chunk_bytes + piece - 1) / piece);
+    src.seg_stride = piece;
+    src.seg_sizes = nullptr;
+    src.const_size = piece;
+    src.const_total = s->chunk_bytes;
+  } else {
+    src.base = s->d_comp;
+    src.chunk_stride = s->chunk_slot;
+    src.seg_stride = s->seg_stride;
+    src.nseg = s->nseg;
+    src.seg_sizes = s->d_seg_sizes;
+  }
+  return src;
+}
+
+// ---- compression of a batch ----
+
+int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
+{
+  const uint64_t n = (uint64_t)n_epochs * s->plan.total_chunks;
+  const auto codec = s->plan.config.codec;
+  if (codec.id == CODEC_NONE || n == 0)
+    return 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  HIPCHECK(hipEventRecord(e0, s->st_comp));
+  if (codec.id == CODEC_LZ4_NON_STANDARD) {
+    HIPCHECK(launch_lz4(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
+                        s->chunk_slot, s->d_seg_sizes, s->lz4_hash_log, s->st_comp));
+  } else {
+    HIPCHECK(launch_zstd(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
+                         s->chunk_slot, s->d_seg_sizes, codec.level, s->d_zscratch, s->zworkers,
+                         s->d_queue, s->st_comp));
+  }
+  HIPCHECK(hipEventRecord(e1, s->st_comp));
+  HIPCHECK(hipEventSynchronize(e1));
+  add_metric(s->metrics.compress, event_ms(e0, e1), (double)n * s->chunk_bytes, 0);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+// ---- aggregate + D2H + delivery of one level ----
+
+int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t* pool_epochs)
+{
+  const LevelInfo& L = s->plan.levels[lv];
+  const uint64_t M = L.agg.chunks_per_epoch, C = L.agg.covering_count;
+  const uint64_t n_items = (uint64_t)active * M, n_slots = (uint64_t)active * C;
+  if (n_items > s->lut_cap || n_slots > s->slot_cap) {
+    set_error("aggregate: batch exceeds the allocated LUT capacity");
+    return 1;
+  }
+  uint32_t* hg = s->h_lut;
+  uint32_t* hp = s->h_lut + s->lut_cap;
+  batch_luts(s->plan, lv, active, pool_epochs, hg, hp);
+  HIPCHECK(hipMemcpyAsync(s->d_gather, hg, n_items * 4, hipMemcpyHostToDevice, s->st_comp));
+  HIPCHECK(hipMemcpyAsync(s->d_perm, hp, n_items * 4, hipMemcpyHostToDevice, s->st_comp));
+
+  hipEvent_t e0, e1, e2;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  HIPCHECK(hipEventCreate(&e2));
+  HIPCHECK(hipEventRecord(e0, s->st_comp));
+  const SegmentSource src = segment_source(s);
+  const uint64_t group = (uint64_t)active * L.agg.cps_inner;
+  HIPCHECK(launch_aggregate_plan(src, s->d_gather, s->d_perm, n_items, n_slots, group,
+                                 s->plan.config.shard_alignment, s->d_chunk_sizes, s->d_offsets,
+                                 s->st_comp));
+  uint64_t* h_off = s->h_sizes;
+  uint64_t* h_cs = s->h_sizes + (s->slot_cap + 1);
+  HIPCHECK(hipMemcpyAsync(h_off, s->d_offsets, (n_slots + 1) * 8, hipMemcpyDeviceToHost, s->st_comp));
+  HIPCHECK(hipMemcpyAsync(h_cs, s->d_chunk_sizes, n_slots * 8, hipMemcpyDeviceToHost, s->st_comp));
+  HIPCHECK(hipStreamSynchronize(s->st_comp));
+  const uint64_t total = h_off[n_slots];
+  const uint64_t copy = total + (s->plan.config.shard_alignment ? s->plan.config.shard_alignment : 0);
+  if (copy > s->agg_cap) {
+    set_error("aggregate: output exceeds the aggregate buffer");
+    return 1;
+  }
+  HIPCHECK(launch_aggregate_gather(src, s->d_gather, s->d_perm, n_items, s->d_offsets, s->d_agg,
+                                   s->st_comp));
+  HIPCHECK(hipEventRecord(e1, s->st_comp));
+  if (s->sink->wait_fence)
+    s->sink->wait_fence(s->sink, (uint8_t)lv, s->io_done[lv]);
+  HIPCHECK(hipMemcpyAsync(s->h_agg, s->d_agg, copy, hipMemcpyDeviceToHost, s->st_comp));
+  HIPCHECK(hipEventRecord(e2, s->st_comp));
+  HIPCHECK(hipEventSynchronize(e2));
+  add_metric(s->metrics.aggregate, event_ms(e0, e1), (double)total, (double)total);
+  add_metric(s->metrics.d2h, event_ms(e1, e2), (double)copy, (double)copy);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  AggView v{ s->h_agg, h_off, h_cs };
+  size_t out_bytes = 0;
+  if (deliver_batch((uint8_t)lv, s->shards[lv], v, active, s->sink, s->plan.config.shard_alignment,
+                    &out_bytes))
+    return 1;
+  const float ms =
+    std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  add_metric(s->metrics.sink, ms, (double)out_bytes, 0);
+  if (s->sink->record_fence)
+    s->io_done[lv] = s->sink->record_fence(s->sink, (uint8_t)lv);
+  return 0;
+}
+
+// Scatter launches are bracketed by events; their durations are collected
+// once the queue has passed them.
+int scatter_begin(tile_stream_gpu* s, hipEvent_t* a)
+{
+  HIPCHECK(hipEventCreate(a));
+  HIPCHECK(hipEventRecord(*a, s->st_comp));
+  return 0;
+}
+
+int scatter_end(tile_stream_gpu* s, hipEvent_t a, double bytes)
+{
+  hipEvent_t b;
+  HIPCHECK(hipEventCreate(&b));
+  HIPCHECK(hipEventRecord(b, s->st_comp));
+  s->pending_scatter.push_back({ a, b, bytes });
+  return 0;
+}
+
+void collect_scatter(tile_stream_gpu* s)
+{
+  for (auto& t : s->pending_scatter) {
+    if (hipEventSynchronize(t.b) == hipSuccess)
+      add_metric(s->metrics.scatter, event_ms(t.a, t.b), t.bytes, t.bytes);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  s->pending_scatter.clear();
+}
+
+// src/cpu/pipeline.c:112-190: compress the batch, then aggregate + deliver
+// every level that has active epochs.
+int flush_batch(tile_stream_gpu* s, uint32_t n_epochs, const uint32_t* masks)
+{
+  if (compress_batch(s, n_epochs))
+    return 1;
+  collect_scatter(s);
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    uint32_t pool_epochs[MAX_BATCH_EPOCHS];
+    uint32_t active = 0;
+    for (uint32_t e = 0; e < n_epochs; ++e)
+      if (masks[e] & (1u << lv))
+        pool_epochs[active++] = e;
+    if (!active)
+      continue;
+    if (aggregate_level(s, lv, active, pool_epochs))
+      return 1;
+  }
+  // every epoch appended so far has reached the sink once L0 is delivered
+  const auto now = std::chrono::steady_clock::now();
+  while (s->epochs_written < s->epochs_done) {
+    const auto& t0 = s->epoch_t0[s->epochs_written % s->epoch_t0.size()];
+    s->latencies.push_back(std::chrono::duration<double>(now - t0).count());
+    ++s->epochs_written;
+  }
+  return 0;
+}
+
+// ---- ingest ----
+
+int dispatch_stage(tile_stream_gpu* s)
+{
+  const size_t bytes = s->stage_fill;
+  if (bytes == 0)
+    return 0;
+  const int i = s->stage_cur;
+  const uint64_t n = bytes / s->plan.bpe;
+  const uint64_t E = epoch_elements(s);
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  HIPCHECK(hipEventRecord(a, s->st_h2d));
+  if (s->plan.enable_multiscale) {
+    HIPCHECK(hipStreamWaitEvent(s->st_h2d, s->ev_linear_free, 0));
+    HIPCHECK(hipMemcpyAsync(s->d_linear + (s->cursor % E) * s->plan.bpe, s->h_stage[i], bytes,
+                            hipMemcpyHostToDevice, s->st_h2d));
+    HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
+    HIPCHECK(hipStreamWaitEvent(s->st_comp, s->ev_h2d[i], 0));
+  } else {
+    HIPCHECK(hipMemcpyAsync(s->d_stage[i], s->h_stage[i], bytes, hipMemcpyHostToDevice, s->st_h2d));
+    HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
+    HIPCHECK(hipStreamWaitEvent(s->st_comp, s->ev_h2d[i], 0));
+    hipEvent_t ta = nullptr;
+    if (scatter_begin(s, &ta))
+      return 1;
+    HIPCHECK(launch_transpose(pool_epoch(s, s->batch_acc), s->d_stage[i], n, (int)s->plan.bpe,
+                              s->cursor, space_of(s->plan.levels[0].layout), s->st_comp));
+    if (scatter_end(s, ta, (double)bytes))
+      return 1;
+  }
+  HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
+  HIPCHECK(hipEventRecord(b, s->st_h2d));
+  s->cursor += n;
+  s->stage_fill = 0;
+  s->stage_cur ^= 1;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  s->metrics.h2d.input_bytes += (double)bytes;
+  s->metrics.h2d.count++;
+  return 0;
+}
+
+int update_metadata(tile_stream_gpu* s, bool final)
+{
+  if (!s->sink->update_append)
+    return 0;
+  const Plan& p = s->plan;
+  const int na = p.n_append;
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    uint64_t sizes[HALF_MAX_RANK] = {};
+    for (int d = 1; d < na; ++d)
+      sizes[d] = p.dims[d].size;
+    if (final) {
+      // src/stream/dim_info.h:77-99
+      uint64_t inner = 1;
+      for (int d = 1; d < p.rank; ++d)
+        inner *= p.dims[d].size;
+      uint64_t dim0 = inner ? s->cursor / inner : 0;
+      if (lv > 0 && p.append_downsample)
+        dim0 = (dim0 + ((1ull << lv) - 1)) >> lv;
+      sizes[0] = dim0;
+    } else {
+      // src/stream/dim_info.c:7-22
+      const ShardState& ss = s->shards[lv];
+      const uint64_t total_ac = ss.shard_epoch * ss.cps_append + ss.epoch_in_shard;
+      sizes[0] = ceildiv(total_ac, p.bounded_append_chunks) * p.dims[0].chunk_size;
+    }
+    if (s->sink->update_append(s->sink, (uint8_t)lv, (uint8_t)na, sizes)) {
+      set_error("update_append failed");
+      return 1;
+    }
+  }
+  return 0;
+}
+
+// Epoch boundary: LOD (multiscale), record the active mask, flush at K.
+int end_epoch(tile_stream_gpu* s)
+{
+  uint32_t mask = 1;
+  if (s->plan.enable_multiscale) {
+    if (lod_run_epoch(s, s->batch_acc, &mask))
+      return 1;
+    HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
+  }
+  if (s->batch_acc >= MAX_BATCH_EPOCHS) {
+    set_error("batch overflow");
+    return 1;
+  }
+  s->masks[s->batch_acc++] = mask;
+  s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
+  s->epochs_done++;
+  if (s->batch_acc == s->plan.epochs_per_batch) {
+    if (flush_batch(s, s->batch_acc, s->masks))
+      return 1;
+    s->batch_acc = 0;
+  }
+  if (s->sink->update_append) {
+    const float elapsed = std::chrono::duration<float>(std::chrono::steady_clock::now() - s->meta_clock).count();
+    if (elapsed >= s->plan.config.metadata_update_interval_s) {
+      s->meta_clock = std::chrono::steady_clock::now();
+      if (update_metadata(s, false))
+        return 1;
+    }
+  }
+  return 0;
+}
+
+int flush_all(tile_stream_gpu* s)
+{
+  if (dispatch_stage(s))
+    return 1;
+  const uint64_t E = epoch_elements(s);
+  if (s->cursor % E != 0) {
+    uint32_t mask = 1;
+    if (s->plan.enable_multiscale && lod_run_epoch(s, s->batch_acc, &mask))
+      return 1;
+    if (s->batch_acc >= MAX_BATCH_EPOCHS)
+      return 1;
+    s->masks[s->batch_acc++] = mask;
+    s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
+    s->epochs_done++;
+  }
+  if (s->batch_acc > 0) {
+    if (flush_batch(s, s->batch_acc, s->masks))
+      return 1;
+    s->batch_acc = 0;
+  }
+  if (s->plan.enable_multiscale && s->plan.append_downsample) {
+    uint32_t drain = 0;
+    if (lod_drain(s, &drain))
+      return 1;
+    if (drain) {
+      uint32_t m[1] = { drain };
+      if (flush_batch(s, 1, m))
+        return 1;
+    }
+  }
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    if (s->sink->wait_fence)
+      s->sink->wait_fence(s->sink, (uint8_t)lv, s->io_done[lv]);
+    if (s->shards[lv].epoch_in_shard > 0 && finalize_shards(s->shards[lv], s->plan.config.shard_alignment))
+      return 1;
+  }
+  return update_metadata(s, true);
+}
+
+// ---- writer vtable ----
+
+template<bool DEVICE>
+writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t* end)
+{
+  if (s->flushed)
+    return writer_finished_at(src, end);
+  if (hipSetDevice(s->device) != hipSuccess)
+    return writer_error_at(src, end);
+  const size_t bpe = s->plan.bpe;
+  const uint64_t E = epoch_elements(s);
+  const uint64_t maxc = s->plan.max_cursor_elements;
+  while (src < end) {
+    if (maxc && s->cursor >= maxc) {
+      const int rc = flush_all(s);
+      s->flushed = 1;
+      return rc ? writer_error_at(src, end) : writer_finished_at(src, end);
+    }
+    const uint64_t in_left = (uint64_t)(end - src) / bpe;
+    if (in_left == 0)
+      break; // trailing partial element stays unconsumed
+    uint64_t elems = std::min<uint64_t>(E - s->cursor % E, in_left);
+    if (maxc)
+      elems = std::min<uint64_t>(elems, maxc - s->cursor);
+    const uint64_t bytes = elems * bpe;
+    if (DEVICE) {
+      const uint64_t off = s->cursor % E;
+      if (s->plan.enable_multiscale) {
+        if (hipMemcpyAsync(s->d_linear + off * bpe, src, bytes, hipMemcpyDeviceToDevice, s->st_comp) != hipSuccess)
+          return writer_error_at(src, end);
+      } else {
+        hipEvent_t ta = nullptr;
+        if (scatter_begin(s, &ta))
+          return writer_error_at(src, end);
+        if (launch_transpose(pool_epoch(s, s->batch_acc), src, elems, (int)bpe, s->cursor,
+                             space_of(s->plan.levels[0].layout), s->st_comp) != hipSuccess) {
+          set_error("transpose launch failed");
+          return writer_error_at(src, end);
+        }
+        if (scatter_end(s, ta, (double)bytes))
+          return writer_error_at(src, end);
+      }
+      s->cursor += elems;
+    } else {
+      uint64_t written = 0;
+      while (written < bytes) {
+        const int i = s->stage_cur;
+        if (s->stage_fill == 0 && hipEventSynchronize(s->ev_stage_free[i]) != hipSuccess)
+          return writer_error_at(src, end);
+        const size_t payload = (size_t)std::min<uint64_t>(s->stage_cap - s->stage_fill, bytes - written);
+        const auto t0 = std::chrono::steady_clock::now();
+        std::memcpy(s->h_stage[i] + s->stage_fill, src + written, payload);
+        add_metric(s->metrics.memcpy,
+                   std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                   (double)payload, (double)payload);
+        s->stage_fill += payload;
+        written += payload;
+        if (s->stage_fill == s->stage_cap || written == bytes)
+          if (dispatch_stage(s))
+            return writer_error_at(src, end);
+      }
+    }
+    src += bytes;
+    if (s->cursor % E == 0 && s->cursor > 0)
+      if (end_epoch(s))
+        return writer_error_at(src, end);
+  }
+  return writer_result{ 0, { src, end } };
+}
+
+writer_result gpu_append(writer* self, slice data)
+{
+  tile_stream_gpu* s = reinterpret_cast<tile_stream_gpu*>(self);
+  return append_impl<false>(s, (const uint8_t*)data.beg, (const uint8_t*)data.end);
+}
+
+writer_result gpu_flush(writer* self)
+{
+  tile_stream_gpu* s = reinterpret_cast<tile_stream_gpu*>(self);
+  if (s->flushed)
+    return writer_finished_at(nullptr, nullptr);
+  if (hipSetDevice(s->device) != hipSuccess)
+    return writer_error();
+  const int rc = flush_all(s);
+  s->flushed = 1;
+  return rc ? writer_error() : writer_ok();
+}
+
+void free_stream(tile_stream_gpu* s)
+{
+  if (!s)
+    return;
+  (void)hipSetDevice(s->device);
+  if (s->st_comp)
+    (void)hipStreamSynchronize(s->st_comp);
+  if (s->st_h2d)
+    (void)hipStreamSynchronize(s->st_h2d);
+  if (s->st_d2h)
+    (void)hipStreamSynchronize(s->st_d2h);
+  if (s->ev_linear_free)
+    (void)hipEventDestroy(s->ev_linear_free);
+  for (int i = 0; i < 2; ++i) {
+    if (s->ev_h2d[i])
+      (void)hipEventDestroy(s->ev_h2d[i]);
+    if (s->ev_stage_free[i])
+      (void)hipEventDestroy(s->ev_stage_free[i]);
+  }
+  for (void* p : s->allocations)
+    (void)hipFree(p);
+  for (void* p : s->host_allocations)
+    (void)hipHostFree(p);
+  if (s->st_comp)
+    (void)hipStreamDestroy(s->st_comp);
+  if (s->st_h2d)
+    (void)hipStreamDestroy(s->st_h2d);
+  if (s->st_d2h)
+    (void)hipStreamDestroy(s->st_d2h);
+  delete s;
+}
+
+uint32_t default_workers()
+{
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return (uint32_t)cus * 8;
+}
+
+int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, shard_sink* sink)
+{
+  std::string err;
+  if (!config || !sink) {
+    set_error("create: null configuration or sink");
+    return 1;
+  }
+  if (!codec_is_gpu_supported(config->codec.id)) {
+    set_error("create: codec not supported on the GPU");
+    return 1;
+  }
+  if (build_plan(config, 1, gpu_max_output, &s->plan, &err)) {
+    set_error("create: " + err);
+    return 1;
+  }
+  const Plan& p = s->plan;
+  s->sink = sink;
+  s->w.append = gpu_append;
+  s->w.flush = gpu_flush;
+  if (s->opts.device >= 0)
+    HIPCHECK(hipSetDevice(s->opts.device));
+  HIPCHECK(hipGetDevice(&s->device));
+  HIPCHECK(hipStreamCreateWithFlags(&s->st_h2d, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&s->st_comp, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&s->st_d2h, hipStreamNonBlocking));
+
+  const uint32_t K = p.epochs_per_batch;
+  const tile_stream_layout& L0 = p.levels[0].layout;
+  s->chunk_stride_bytes = L0.chunk_stride * p.bpe;
+  s->chunk_bytes = L0.chunk_elements * p.bpe;
+  s->epoch_pool_bytes = p.total_chunks * s->chunk_stride_bytes;
+
+  HIPCHECK(hipEventCreateWithFlags(&s->ev_linear_free, hipEventDisableTiming));
+  HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
+
+  // staging
+  s->stage_cap = std::max<size_t>(align_up(config->buffer_capacity_bytes, 4096), 4096);
+  s->stage_cap -= s->stage_cap % p.bpe;
+  for (int i = 0; i < 2; ++i) {
+    if (halloc(s, (void**)&s->h_stage[i], s->stage_cap))
+      return 1;
+    if (!p.enable_multiscale && dalloc(s, (void**)&s->d_stage[i], s->stage_cap))
+      return 1;
+    HIPCHECK(hipEventCreateWithFlags(&s->ev_h2d[i], hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&s->ev_stage_free[i], hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
+  }
+
+  // pool (zeroed once; padding positions stay zero for the stream's life)
+  if (dalloc(s, (void**)&s->d_pool, (uint64_t)K * s->epoch_pool_bytes))
+    return 1;
+  HIPCHECK(hipMemsetAsync(s->d_pool, 0, (uint64_t)K * s->epoch_pool_bytes + kSlack, s->st_comp));
+
+  // codec
+  const uint64_t nchunks = (uint64_t)K * p.total_chunks;
+  if (config->codec.id == CODEC_LZ4_NON_STANDARD) {
+    s->nseg = 1;
+    s->chunk_slot = align_up(lz4_max_output_size(s->chunk_bytes), 256);
+    s->seg_stride = s->chunk_slot;
+    s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : 12;
+  } else if (config->codec.id == CODEC_ZSTD) {
+    s->nseg = zstd_segments(s->chunk_bytes);
+    s->seg_stride = zstd_segment_stride();
+    s->chunk_slot = zstd_max_output_size(s->chunk_bytes);
+    s->zworkers = s->opts.compress_workers ? s->opts.compress_workers : default_workers();
+    const uint64_t blocks = nchunks * s->nseg;
+    if (s->zworkers > blocks)
+      s->zworkers = (uint32_t)blocks;
+    if (dalloc(s, &s->d_zscratch, zstd_scratch_bytes(s->zworkers)))
+      return 1;
+    if (dalloc(s, (void**)&s->d_queue, 64))
+      return 1;
+  }
+  if (config->codec.id != CODEC_NONE) {
+    if (dalloc(s, (void**)&s->d_comp, nchunks * s->chunk_slot))
+      return 1;
+    if (dalloc(s, (void**)&s->d_seg_sizes, nchunks * s->nseg * 4))
+      return 1;
+  }
+
+  // aggregate buffers sized for the largest level batch
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    const LevelInfo& L = p.levels[lv];
+    const uint64_t slots = std::max<uint32_t>(L.batch_active_count, 1);
+    const uint64_t kk = std::max<uint64_t>(slots, K); // partial batches recompute with <= K
+    s->lut_cap = std::max<uint64_t>(s->lut_cap, kk * L.agg.chunks_per_epoch);
+    s->slot_cap = std::max<uint64_t>(s->slot_cap, kk * L.agg.covering_count);
+    const uint64_t pad_total = config->shard_alignment
+                                 ? (kk * L.agg.covering_count / std::max<uint64_t>(L.agg.cps_inner, 1) + 2) *
+                                     config->shard_alignment
+                                 : 0;
+    s->agg_cap = std::max<uint64_t>(s->agg_cap, kk * L.agg.chunks_per_epoch * p.max_output_size + pad_total);
+  }
+  if (dalloc(s, (void**)&s->d_gather, s->lut_cap * 4) || dalloc(s, (void**)&s->d_perm, s->lut_cap * 4) ||
+      halloc(s, (void**)&s->h_lut, s->lut_cap * 8) ||
+      dalloc(s, (void**)&s->d_chunk_sizes, s->slot_cap * 8) ||
+      dalloc(s, (void**)&s->d_offsets, (s->slot_cap + 1) * 8) ||
+      halloc(s, (void**)&s->h_sizes, (2 * s->slot_cap + 1) * 8) ||
+      dalloc(s, (void**)&s->d_agg, s->agg_cap) || halloc(s, (void**)&s->h_agg, s->agg_cap))
+    return 1;
+
+  // shards
+  s->shards.resize(p.nlod);
+  for (int lv = 0; lv < p.nlod; ++lv)
+    s->shards[lv].init(p.levels[lv]);
+  s->io_done.assign(p.nlod, io_event{ 0 });
+
+  // LOD
+  if (p.enable_multiscale && lod_init(s))
+    return 1;
+
+  // device copy of the L0 lifted layout
+  if (dalloc(s, (void**)&s->layout_gpu.d_lifted_shape, MAX_RANK * 8) ||
+      dalloc(s, (void**)&s->layout_gpu.d_lifted_strides, MAX_RANK * 8))
+    return 1;
+  HIPCHECK(hipMemcpy(s->layout_gpu.d_lifted_shape, L0.lifted_shape, MAX_RANK * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(s->layout_gpu.d_lifted_strides, L0.lifted_strides, MAX_RANK * 8, hipMemcpyHostToDevice));
+
+  s->metrics.memcpy = mk_metric("memcpy");
+  s->metrics.h2d = mk_metric("h2d");
+  s->metrics.scatter = mk_metric("scatter");
+  s->metrics.compress = mk_metric("compress");
+  s->metrics.aggregate = mk_metric("aggregate");
+  s->metrics.d2h = mk_metric("d2h");
+  s->metrics.sink = mk_metric("sink");
+  s->metrics.lod_gather = mk_metric("lod_gather");
+  s->metrics.lod_reduce = mk_metric("lod_reduce");
+  s->metrics.lod_append_fold = mk_metric("lod_append_fold");
+  s->metrics.lod_morton_chunk = mk_metric("lod_morton");
+  s->meta_clock = std::chrono::steady_clock::now();
+  s->epoch_t0.resize(MAX_BATCH_EPOCHS * 2);
+  HIPCHECK(hipStreamSynchronize(s->st_comp));
+  return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o)
+{
+  *o = tile_stream_gpu_options{};
+  o->pipeline = CHUCKY_PIPELINE_PLAIN;
+  o->device = -1;
+}
+
+struct tile_stream_gpu* tile_stream_gpu_create_ex(const struct tile_stream_configuration* config,
+                                                  struct shard_sink* sink,
+                                                  const struct tile_stream_gpu_options* opts)
+{
+  tile_stream_gpu* s = new (std::nothrow) tile_stream_gpu();
+  if (!s)
+    return nullptr;
+  if (opts)
+    s->opts = *opts;
+  else
+    tile_stream_gpu_options_default(&s->opts);
+  s->pipeline = s->opts.pipeline;
+  if (create_impl(s, config, sink)) {
+    free_stream(s);
+    return nullptr;
+  }
+  return s;
+}
+
+struct tile_stream_gpu* tile_stream_gpu_create(const struct tile_stream_configuration* config,
+                                               struct shard_sink* sink)
+{
+  return tile_stream_gpu_create_ex(config, sink, nullptr);
+}
+
+void tile_stream_gpu_destroy(struct tile_stream_gpu* stream) { free_stream(stream); }
+
+struct stream_metrics tile_stream_gpu_get_metrics(const struct tile_stream_gpu* s) { return s->metrics; }
+
+const struct tile_stream_layout* tile_stream_gpu_layout(const struct tile_stream_gpu* s)
+{
+  return &s->plan.levels[0].layout;
+}
+
+struct writer* tile_stream_gpu_writer(struct tile_stream_gpu* s) { return &s->w; }
+
+uint64_t tile_stream_gpu_cursor(const struct tile_stream_gpu* s) { return s->cursor; }
+
+struct tile_stream_status tile_stream_gpu_status(const struct tile_stream_gpu* s)
+{
+  tile_stream_status st{};
+  st.nlod = s->plan.nlod;
+  st.append_downsample = s->plan.append_downsample;
+  st.epochs_per_batch = s->plan.epochs_per_batch;
+  st.max_compressed_size = s->plan.max_output_size;
+  st.dtype = s->plan.config.dtype;
+  st.codec = s->plan.config.codec;
+  st.codec_batch_size = (size_t)s->plan.epochs_per_batch * s->plan.total_chunks;
+  st.batch_accumulated = s->batch_acc;
+  st.pool_current = 0;
+  st.flush_pending = 0;
+  return st;
+}
+
+struct writer_result tile_stream_gpu_append_device(struct tile_stream_gpu* s, const void* d_src,
+                                                   size_t nbytes)
+{
+  const uint8_t* b = (const uint8_t*)d_src;
+  return append_impl<true>(s, b, b + nbytes);
+}
+
+int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
+{
+  if (hipSetDevice(s->device) != hipSuccess || hipStreamSynchronize(s->st_comp) != hipSuccess)
+    return 1;
+  if (sink)
+    s->sink = sink;
+  s->cursor = 0;
+  s->batch_acc = 0;
+  s->flushed = 0;
+  s->stage_fill = 0;
+  std::fill(std::begin(s->append_counts), std::end(s->append_counts), 0u);
+  for (auto& ss : s->shards)
+    ss.reset();
+  std::fill(s->io_done.begin(), s->io_done.end(), io_event{ 0 });
+  s->latencies.clear();
+  s->epochs_done = s->epochs_written = 0;
+  return 0;
+}
+
+int tile_stream_gpu_level_geometry(const struct tile_stream_gpu* s, int level,
+                                   uint64_t* cps_total, uint64_t* shard_inner_count,
+                                   uint64_t* chunks_per_epoch)
+{
+  if (!s || level < 0 || level >= s->plan.nlod)
+    return 1;
+  const LevelInfo& L = s->plan.levels[level];
+  if (cps_total)
+    *cps_total = L.cps_total;
+  if (shard_inner_count)
+    *shard_inner_count = L.shard_inner_count;
+  if (chunks_per_epoch)
+    *chunks_per_epoch = L.chunk_count;
+  return 0;
+}
+
+size_t tile_stream_gpu_epoch_latencies(const struct tile_stream_gpu* s, double* out, size_t cap)
+{
+  const size_t n = std::min(cap, s->latencies.size());
+  for (size_t i = 0; i < n; ++i)
+    out[i] = s->latencies[i];
+  return n;
+}
+
+int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* config,
+                                    struct tile_stream_memory_info* info)
+{
+  if (!info || !config)
+    return 1;
+  *info = tile_stream_memory_info{};
+  Plan p;
+  std::string err;
+  if (build_plan(config, 1, gpu_max_output, &p, &err))
+    return 1;
+  const uint32_t K = p.epochs_per_batch;
+  const uint64_t csb = p.levels[0].layout.chunk_stride * p.bpe;
+  const uint64_t cb = p.levels[0].layout.chunk_elements * p.bpe;
+  const uint64_t nchunks = (uint64_t)K * p.total_chunks;
+  info->staging_bytes = 2 * align_up(config->buffer_capacity_bytes, 4096);
+  info->chunk_pool_bytes = nchunks * csb;
+  info->compressed_pool_bytes = config->codec.id == CODEC_NONE ? 0 : nchunks * p.max_output_size;
+  uint64_t agg = 0;
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    const LevelInfo& L = p.levels[lv];
+    const uint64_t kk = std::max<uint64_t>(std::max<uint32_t>(L.batch_active_count, 1), K);
+    agg = std::max<uint64_t>(agg, kk * L.agg.chunks_per_epoch * p.max_output_size);
+  }
+  info->aggregate_bytes = agg;
+  uint64_t lod = 0;
+  if (p.enable_multiscale) {
+    lod += p.levels[0].layout.epoch_elements * p.bpe;
+    lod += p.level_span_ends[p.nlod - 1] * p.bpe;
+    lod += p.append_accum_elements() * p.bpe;
+    lod += p.ends.size() * 8;
+    for (int lv = 0; lv < p.nlod; ++lv)
+      lod += p.levels[lv].lod_nelem * 4 + p.fixed_count * 8;
+  }
+  info->lod_bytes = lod;
+  if (config->codec.id == CODEC_ZSTD) {
+    const uint64_t blocks = nchunks * zstd_segments(cb);
+    info->codec_bytes = zstd_scratch_bytes((uint32_t)std::min<uint64_t>(blocks, 2048));
+  }
+  info->codec_bytes += nchunks * (config->codec.id == CODEC_ZSTD ? zstd_segments(cb) : 1) * 4;
+  uint64_t shard = 0;
+  for (int lv = 0; lv < p.nlod; ++lv)
+    shard += p.levels[lv].shard_inner_count * (sizeof(OpenShard) + p.levels[lv].cps_total * 16);
+  info->shard_bytes = shard;
+  info->device_bytes = info->staging_bytes + info->chunk_pool_bytes + info->compressed_pool_bytes +
+                       info->aggregate_bytes + info->lod_bytes + info->codec_bytes;
+  info->host_pinned_bytes = info->staging_bytes + info->aggregate_bytes;
+  info->chunks_per_epoch = p.levels[0].layout.chunks_per_epoch;
+  info->total_chunks = p.total_chunks;
+  info->max_output_size = p.max_output_size;
+  info->nlod = p.nlod;
+  info->epochs_per_batch = K;
+  return 0;
+}
+
+int tile_stream_gpu_advise_chunk_sizes(struct tile_stream_configuration* config,
+                                       size_t target_chunk_bytes, const uint8_t* ratios,
+                                       size_t budget_bytes)
+{
+  const size_t bpe = dtype_bpe(config->dtype);
+  if (bpe == 0 || budget_bytes == 0)
+    return 1;
+  for (size_t target = target_chunk_bytes; target >= bpe; target >>= 1) {
+    dims_budget_chunk_bytes(config->dimensions, config->rank, target, bpe, ratios);
+    tile_stream_memory_info mem;
+    if (tile_stream_gpu_memory_estimate(config, &mem))
+      return 1;
+    if (mem.device_bytes <= budget_bytes)
+      return 0;
+  }
+  return 1;
+}
+
+} // extern "C"

@@ -1,0 +1,111 @@
+// Internal state of a GPU stream (both pipelines).
+#pragma once
+
+#include "chucky_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "shards.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+struct tile_stream_gpu
+{
+  writer w; // must stay the first member (writer* <-> stream*)
+  chucky::Plan plan;
+  chucky::LodLuts luts;
+  shard_sink* sink = nullptr;
+  tile_stream_gpu_options opts{};
+  int device = 0;
+  int pipeline = CHUCKY_PIPELINE_PLAIN;
+
+  hipStream_t st_h2d = nullptr, st_comp = nullptr, st_d2h = nullptr;
+
+  // host staging ring (host-memory appends)
+  size_t stage_cap = 0;
+  uint8_t* h_stage[2] = {};
+  uint8_t* d_stage[2] = {};
+  hipEvent_t ev_h2d[2] = {}, ev_stage_free[2] = {};
+  hipEvent_t ev_linear_free = nullptr; // LOD gather of the last epoch has read d_linear
+  size_t stage_fill = 0;
+  int stage_cur = 0;
+
+  // chunk pool: K epochs x total_chunks x chunk_stride elements
+  uint8_t* d_pool = nullptr;
+  uint64_t epoch_pool_bytes = 0;
+  uint64_t chunk_stride_bytes = 0;
+  uint64_t chunk_bytes = 0;
+
+  // compressed slots
+  uint8_t* d_comp = nullptr;
+  uint32_t* d_seg_sizes = nullptr;
+  uint32_t nseg = 1;
+  uint64_t seg_stride = 0, chunk_slot = 0;
+  void* d_zscratch = nullptr;
+  uint32_t zworkers = 0;
+  uint32_t* d_queue = nullptr;
+  int lz4_hash_log = 12;
+
+  // aggregate
+  uint32_t* d_gather = nullptr;
+  uint32_t* d_perm = nullptr;
+  uint32_t* h_lut = nullptr; // pinned: gather then perm
+  uint64_t lut_cap = 0;
+  uint64_t* d_chunk_sizes = nullptr;
+  uint64_t* d_offsets = nullptr;
+  uint64_t* h_sizes = nullptr; // pinned: offsets (slots+1) then chunk sizes (slots)
+  uint64_t slot_cap = 0;
+  uint8_t* d_agg = nullptr;
+  uint8_t* h_agg = nullptr;
+  uint64_t agg_cap = 0;
+
+  // LOD (multiscale)
+  uint8_t* d_linear = nullptr;
+  uint8_t* d_lod = nullptr;     // Morton values, all levels packed
+  uint8_t* d_accum = nullptr;   // append-dim accumulators, levels >= 1
+  uint32_t* d_gather_lut = nullptr;
+  uint64_t* d_gather_fixed = nullptr;
+  uint64_t* d_ends = nullptr;
+  std::vector<uint32_t*> d_chunk_lut;
+  std::vector<uint64_t*> d_chunk_fixed;
+  uint32_t append_counts[LOD_MAX_LEVELS] = {};
+
+  // stream state
+  uint64_t cursor = 0;
+  uint32_t batch_acc = 0;
+  uint32_t masks[MAX_BATCH_EPOCHS] = {};
+  int flushed = 0;
+  std::vector<chucky::ShardState> shards;
+  std::vector<io_event> io_done;
+  stream_metrics metrics{};
+  tile_stream_layout_gpu layout_gpu{};
+  std::chrono::steady_clock::time_point meta_clock;
+
+  // latency bookkeeping: completion time of each epoch's append, and the
+  // latency samples once its shards are written
+  std::vector<std::chrono::steady_clock::time_point> epoch_t0;
+  std::vector<double> latencies;
+  uint64_t epochs_done = 0;     // epochs whose append completed
+  uint64_t epochs_written = 0;  // epochs whose shards have been written
+
+  // (start, end, bytes) event pairs of scatter launches not yet accounted
+  struct PendingTiming
+  {
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<PendingTiming> pending_scatter;
+
+  std::vector<void*> allocations;      // device
+  std::vector<void*> host_allocations; // pinned host
+};
+
+namespace chucky {
+void set_error(const std::string& msg);
+int lod_run_epoch(tile_stream_gpu* s, uint32_t epoch_in_batch, uint32_t* out_mask);
+int lod_drain(tile_stream_gpu* s, uint32_t* out_mask);
+int lod_init(tile_stream_gpu* s);
+} // namespace chucky

@@ -1,0 +1,105 @@
+// Wavefront (64-lane) helpers shared by the codec kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace chucky {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// 4 bytes at any address, from two aligned dword loads. Reads up to 3 bytes
+// past p+4: every codec input buffer carries >= 64 bytes of slack.
+__device__ __forceinline__ uint32_t ld32u(const uint8_t* p)
+{
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+  const uint32_t sh = (uint32_t)(a & 3) * 8;
+  const uint32_t lo = w[0];
+  if (sh == 0)
+    return lo;
+  return (uint32_t)((((uint64_t)w[1] << 32) | lo) >> sh);
+}
+
+// Wave-cooperative copy of n bytes (any alignment): byte head to a dword
+// boundary of dst, dword body (funnel-shifted source), byte tail.
+__device__ __forceinline__ void wave_copy(uint8_t* dst, const uint8_t* src, uint32_t n)
+{
+  const int lane = lane_id();
+  const uint32_t head = (uint32_t)((4 - ((uintptr_t)dst & 3)) & 3);
+  const uint32_t h = head < n ? head : n;
+  if ((uint32_t)lane < h)
+    dst[lane] = src[lane];
+  const uint32_t body = (n - h) >> 2;
+  uint32_t* dw = (uint32_t*)(dst + h);
+  const uint8_t* s = src + h;
+  for (uint32_t w = lane; w < body; w += kWave)
+    dw[w] = ld32u(s + 4 * w);
+  const uint32_t t0 = h + body * 4;
+  if ((uint32_t)lane < n - t0)
+    dst[t0 + lane] = src[t0 + lane];
+}
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+template<typename T>
+__device__ __forceinline__ T wave_bcast(T v, int src_lane)
+{
+  return __shfl(v, src_lane, kWave);
+}
+
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+  const int lane = lane_id();
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, kWave);
+    if (lane >= off)
+      v += u;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Number of leading equal bytes (0..4) of two little-endian dwords.
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t a, uint32_t b)
+{
+  const uint32_t x = a ^ b;
+  return x ? (uint32_t)(__builtin_ctz(x) >> 3) : 4u;
+}
+
+// Extend a match forward: bytes equal at p+len.. and c+len.. up to limit
+// (exclusive end of p's range). Returns the final length. Uniform result.
+__device__ __forceinline__ uint32_t wave_match_extend(const uint8_t* base, uint32_t p, uint32_t c,
+                                                      uint32_t len, uint32_t limit)
+{
+  const int lane = lane_id();
+  for (;;) {
+    const uint32_t q = p + len + 4u * lane;
+    uint32_t nb = 0;
+    if (q < limit) {
+      nb = eq_bytes(ld32u(base + q), ld32u(base + c + len + 4u * lane));
+      if (nb > limit - q)
+        nb = limit - q;
+    }
+    const uint64_t short_mask = wave_ballot(nb < 4);
+    if (short_mask == 0) {
+      len += 4u * kWave;
+      continue;
+    }
+    const int first = __ffsll((unsigned long long)short_mask) - 1;
+    const uint32_t nb_first = wave_bcast(nb, first);
+    return len + 4u * (uint32_t)first + nb_first;
+  }
+}
+
+} // namespace chucky

@@ -1,0 +1,130 @@
+/* Entry points of chucky-amd that have no counterpart in the reference.
+ *
+ * They exist for callers the reference API cannot serve on this hardware:
+ * producers whose frames are already in device memory, benchmarks that reuse
+ * one stream across acquisitions, the batched (graph-captured) pipeline, the
+ * multi-stream serving loop, in-memory / discard sinks for tests and benches,
+ * and kernel-level entry points used by parity tests and the autotuner. */
+#pragma once
+
+#include "stream.gpu.h"
+#include "writer.h"
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- stream options ------------------------------------------------------ */
+
+enum chucky_pipeline
+{
+  CHUCKY_PIPELINE_PLAIN = 0,   /* one launch per stage per epoch, host syncs like the reference */
+  CHUCKY_PIPELINE_BATCHED = 1, /* per-epoch hipGraph, persistent compressor, no host sync on the device path */
+};
+
+struct tile_stream_gpu_options
+{
+  int pipeline;              /* enum chucky_pipeline */
+  int device;                /* HIP device ordinal; -1 = current device */
+  uint32_t compress_workers; /* persistent compressor workgroups; 0 = autotuned */
+  uint32_t lz4_hash_log;     /* LDS hash table log2 entries; 0 = autotuned */
+  uint32_t zstd_block_bytes; /* zstd block size (<= 131072); 0 = autotuned */
+};
+
+void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);
+
+struct tile_stream_gpu* tile_stream_gpu_create_ex(const struct tile_stream_configuration* config,
+                                                  struct shard_sink* sink,
+                                                  const struct tile_stream_gpu_options* opts);
+
+/* Append nbytes that already live in device memory (no staging copy).
+ * Same epoch / capacity / finished semantics as writer->append. */
+struct writer_result tile_stream_gpu_append_device(struct tile_stream_gpu* s, const void* d_src,
+                                                   size_t nbytes);
+
+/* Re-arm a flushed stream for a new acquisition into `sink` (NULL = same sink),
+ * keeping every allocation. 0 on success. */
+int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink);
+
+/* Latency record of the most recent epochs: for each finished epoch, seconds
+ * from the append call that completed it to the return of the last shard write
+ * covering it. Returns the number of samples copied (<= cap). */
+size_t tile_stream_gpu_epoch_latencies(const struct tile_stream_gpu* s, double* out, size_t cap);
+
+/* Shard geometry of one level: chunks per shard (all dims), inner shards per
+ * append step, chunks per epoch. 0 on success. */
+int tile_stream_gpu_level_geometry(const struct tile_stream_gpu* s, int level,
+                                   uint64_t* chunks_per_shard_total, uint64_t* shard_inner_count,
+                                   uint64_t* chunks_per_epoch);
+
+/* Last error message of this thread ("" if none). */
+const char* chucky_last_error(void);
+
+/* ---- in-memory and discard sinks ----------------------------------------- */
+
+struct chucky_mem_sink;
+struct chucky_mem_sink* chucky_mem_sink_create(void);
+void chucky_mem_sink_destroy(struct chucky_mem_sink* s);
+struct shard_sink* chucky_mem_sink_as_shard_sink(struct chucky_mem_sink* s);
+/* Number of shards opened so far (all levels). */
+size_t chucky_mem_sink_count(const struct chucky_mem_sink* s);
+/* i-th shard in open order. data stays valid until the sink is destroyed. */
+int chucky_mem_sink_get(const struct chucky_mem_sink* s, size_t i, uint8_t* level,
+                        uint64_t* shard_index, const uint8_t** data, size_t* size,
+                        int* finalized);
+/* Last append extents published through update_append for a level (n values). */
+int chucky_mem_sink_append_sizes(const struct chucky_mem_sink* s, uint8_t level,
+                                 uint64_t* out, uint8_t* n);
+/* Make open() fail (fail_open) or write() fail (fail_write) from now on. */
+void chucky_mem_sink_inject_failure(struct chucky_mem_sink* s, int fail_open, int fail_write);
+
+struct chucky_discard_sink;
+struct chucky_discard_sink* chucky_discard_sink_create(void);
+void chucky_discard_sink_destroy(struct chucky_discard_sink* s);
+struct shard_sink* chucky_discard_sink_as_shard_sink(struct chucky_discard_sink* s);
+uint64_t chucky_discard_sink_bytes(const struct chucky_discard_sink* s);
+uint64_t chucky_discard_sink_finalized(const struct chucky_discard_sink* s);
+
+/* ---- kernel-level entry points (device pointers, synchronous) ------------- */
+
+/* Worst-case compressed bytes of one chunk for a GPU codec. */
+size_t chucky_gpu_max_output_size(enum compression_codec codec, size_t chunk_bytes);
+
+/* Compress n chunks: chunk i is chunk_bytes at d_src + i*src_stride, output at
+ * d_dst + i*dst_stride (dst_stride >= chucky_gpu_max_output_size), sizes into
+ * d_sizes[i] (uint64). 0 on success. */
+int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_src,
+                        size_t src_stride, size_t chunk_bytes, uint64_t n, void* d_dst,
+                        size_t dst_stride, uint64_t* d_sizes);
+
+/* Scatter n elements (element index i_offset + i of the lifted row-major space)
+ * into a chunk pool: dst[ravel(shape, strides, i_offset+i)] = src[i]. */
+int chucky_gpu_transpose(void* d_dst, const void* d_src, uint64_t n, uint8_t bpe,
+                         uint64_t i_offset, uint8_t lifted_rank, const uint64_t* lifted_shape,
+                         const int64_t* lifted_strides);
+
+/* CRC-32C (Castagnoli, reflected, init/xorout 0xFFFFFFFF) of host bytes. */
+uint32_t chucky_crc32c(const void* data, size_t len);
+
+/* ---- device memory helpers (current device, synchronous) ----------------- */
+
+void* chucky_device_alloc(size_t bytes);
+void chucky_device_free(void* p);
+int chucky_memcpy_to_device(void* d_dst, const void* h_src, size_t n);
+int chucky_memcpy_to_host(void* h_dst, const void* d_src, size_t n);
+int chucky_device_synchronize(void);
+
+/* Synthetic 12-bit sensor frames (u16, frames x height x width, row-major) in
+ * device memory: per pixel a Poisson(lambda) background drawn by inversion of
+ * the CDF from a counter-based generator keyed by (seed, pixel index), plus
+ * Gaussian blobs. blobs (host) holds frames*nblob records of 4 floats
+ * (y, x, sigma, amplitude); the sum is rounded and clipped to [0, 4095]. */
+int chucky_synth_sensor_frames(void* d_dst, uint64_t frames, uint32_t height, uint32_t width,
+                               uint64_t seed, double lambda, const float* blobs, uint32_t nblob);
+
+#ifdef __cplusplus
+}
+#endif

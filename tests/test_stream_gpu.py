@@ -1,0 +1,150 @@
+"""GPU parity: the HIP write path against the reference's own CPU path
+(oracle/_ref, built from the reference sources) on the same inputs.
+
+CODEC_NONE shards must be byte-identical (the reference's definition of
+parity, tests/test_cross_validate.c:168-622). With lz4/zstd the compressed
+bytes may differ; every shard's index layout (which slots are filled), CRC and
+every decoded chunk must match.
+"""
+import numpy as np
+import pytest
+
+import chucky_amd as ca
+from chucky_amd.api import Dim
+from oracle import ref
+from tests.helpers import (decoded_chunks, parse_shard, random_typed, run_gpu, run_ref,
+                           thirds_volume, xor_volume)
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")]
+
+
+def _geom(cfg):
+    s = ca.GpuStream(cfg, ca.MemSink())
+    n = s.status().nlod
+    g = {lv: s.level_geometry(lv) for lv in range(n)}
+    E = s.epoch_elements
+    chunk_bytes = s.layout.chunk_elements * cfg.np_dtype.itemsize
+    s.close()
+    return g, E, chunk_bytes
+
+
+def _compare(cfg, data, pieces=1, device_input=False, pipeline=0):
+    g, _, chunk_bytes = _geom(cfg)
+    gs, gmeta, gcur = run_gpu(cfg, data, pieces=pieces, device_input=device_input,
+                              pipeline=pipeline)
+    rs, rmeta, rcur = run_ref(cfg, data, pieces=pieces)
+    assert gcur == rcur
+    assert set(gs) == set(rs), (sorted(gs), sorted(rs))
+    for lv in gmeta:
+        assert gmeta[lv] == rmeta[lv], (lv, gmeta[lv], rmeta[lv])
+    if cfg.codec == "none":
+        for k in rs:
+            assert gs[k] == rs[k], f"shard {k} differs"
+        return gs
+    cps = lambda lv: g[lv][0]  # noqa: E731
+    for k in rs:
+        gi = [e is None for e in parse_shard(gs[k], cps(k[0]))]
+        ri = [e is None for e in parse_shard(rs[k], cps(k[0]))]
+        assert gi == ri, f"index layout of shard {k} differs"
+    gd = decoded_chunks(gs, cps, cfg.codec, chunk_bytes)
+    rd = decoded_chunks(rs, cps, cfg.codec, chunk_bytes)
+    assert gd.keys() == rd.keys()
+    for k in rd:
+        assert gd[k] == rd[k], f"decoded chunk {k} differs"
+    return gs
+
+
+def test_cross_validate_basic():
+    # tests/test_cross_validate.c:168-252: 4x4x6, chunk 2x2x3, 2 epochs
+    cfg = ca.Config([Dim(4, 2, 2), Dim(4, 2, 2), Dim(6, 3, 2)], dtype="u16",
+                    buffer_capacity_bytes=4096)
+    _, E, _ = _geom(cfg)
+    data = (np.arange(2 * E) % 65536).astype(np.uint16)
+    _compare(cfg, data)
+
+
+def test_cross_validate_multishard():
+    # tests/test_cross_validate.c:254-330: 6x8x9, chunk 2x4x3, cps 1
+    cfg = ca.Config([Dim(6, 2, 1), Dim(8, 4, 1), Dim(9, 3, 1)], dtype="u16",
+                    buffer_capacity_bytes=8192)
+    _, E, _ = _geom(cfg)
+    data = random_typed(3 * E, "u16", seed=3)
+    _compare(cfg, data)
+
+
+@pytest.mark.parametrize("codec", ["none", "lz4", "zstd"])
+def test_volume_codecs(codec):
+    v = xor_volume((64, 64, 64))
+    cfg = ca.Config([Dim(64, 16, 2), Dim(64, 16, 2), Dim(64, 16, 4)], dtype="u16", codec=codec,
+                    buffer_capacity_bytes=96 * 1024)
+    _compare(cfg, v, pieces=5)
+
+
+@pytest.mark.parametrize("codec", ["lz4", "zstd"])
+def test_thirds_codecs(codec):
+    v = thirds_volume((32, 96, 192))
+    cfg = ca.Config([Dim(32, 16, 0), Dim(96, 32, 2), Dim(192, 64, 0)], dtype="u16", codec=codec)
+    _compare(cfg, v, pieces=2)
+
+
+def test_partial_last_epoch_and_small_buffer():
+    cfg = ca.Config([Dim(20, 8, 2), Dim(16, 8, 1), Dim(24, 8, 3)], dtype="u8",
+                    buffer_capacity_bytes=4096)
+    data = random_typed(20 * 16 * 24, "u8", seed=11)
+    _compare(cfg, data, pieces=7)
+
+
+def test_device_input_matches_host_input():
+    v = xor_volume((32, 64, 64))
+    cfg = ca.Config([Dim(32, 8, 2), Dim(64, 32, 1), Dim(64, 32, 2)], dtype="u16", codec="zstd")
+    _compare(cfg, v, pieces=3, device_input=True)
+
+
+@pytest.mark.parametrize("dtype", ["u8", "u32", "i16", "f32", "f64", "u64"])
+def test_dtypes_none(dtype):
+    cfg = ca.Config([Dim(12, 4, 0), Dim(10, 5, 1), Dim(16, 8, 0)], dtype=dtype)
+    data = random_typed(12 * 10 * 16, dtype, seed=5)
+    _compare(cfg, data, pieces=2)
+
+
+def test_storage_order_permuted():
+    # storage order z, x, y (the append dim stays first)
+    cfg = ca.Config([Dim(8, 4, 1), Dim(12, 4, 0, storage_position=2), Dim(16, 8, 0, storage_position=1)],
+                    dtype="u16")
+    data = random_typed(8 * 12 * 16, "u16", seed=9)
+    _compare(cfg, data)
+
+
+def test_shard_alignment_padding():
+    cfg = ca.Config([Dim(16, 4, 2), Dim(32, 16, 1), Dim(32, 16, 2)], dtype="u16", codec="lz4",
+                    shard_alignment=4096)
+    v = xor_volume((16, 32, 32))
+    _compare(cfg, v)
+
+
+@pytest.mark.parametrize("reduce", ["mean", "min", "max", "median", "max_sup", "min_sup"])
+def test_multiscale_inner(reduce):
+    # LOD over y, x without dim0 downsampling: all levels every epoch (CPU path)
+    cfg = ca.Config([Dim(16, 8, 1), Dim(48, 8, 0, downsample=True), Dim(40, 8, 0, downsample=True)],
+                    dtype="u16", reduce=reduce)
+    data = random_typed(16 * 48 * 40, "u16", seed=21)
+    _compare(cfg, data, pieces=3)
+
+
+@pytest.mark.parametrize("dtype", ["u8", "i8", "u16", "i32", "u32", "f32", "f64", "i64"])
+@pytest.mark.parametrize("append_reduce", ["mean", "min", "max"])
+def test_multiscale_dim0(dtype, append_reduce):
+    # append dim downsampled: cross-epoch fold with the level-shift mean
+    cfg = ca.Config([Dim(40, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
+                     Dim(20, 4, 0, downsample=True)],
+                    dtype=dtype, append_reduce=append_reduce, reduce="mean")
+    data = random_typed(40 * 24 * 20, dtype, seed=33)
+    _compare(cfg, data, pieces=4)
+
+
+def test_multiscale_dim0_zstd():
+    v = xor_volume((64, 64, 64))
+    cfg = ca.Config([Dim(64, 16, 2, downsample=True), Dim(64, 16, 0, downsample=True),
+                     Dim(64, 16, 0, downsample=True)], dtype="u16", codec="zstd")
+    _compare(cfg, v, pieces=2)
