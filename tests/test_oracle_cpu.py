@@ -1,0 +1,155 @@
+"""Pins the numpy restatement (oracle/chucky_oracle.py) on the CPU:
+against the reference's own CPU path built from its sources (oracle/_ref),
+against the committed golden shard files, and against closed forms."""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import chucky_amd as ca
+from chucky_amd.api import Dim
+from oracle import chucky_oracle as O
+from oracle import ref
+from tests.helpers import random_typed, run_ref
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+needs_ref = pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
+
+
+def test_crc32c_check_value():
+    # standard CRC-32C check value (Castagnoli): "123456789" -> 0xE3069283
+    assert O.crc32c(b"123456789") == 0xE3069283
+    assert ca.crc32c(b"123456789") == 0xE3069283
+    rng = np.random.default_rng(1)
+    for n in (0, 1, 7, 8, 9, 63, 1000):
+        b = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        assert O.crc32c(b) == ca.crc32c(b)
+
+
+def test_morton_table_3x5():
+    # compacted Morton order of a 3x5 box (docs/design.md): rows y, cols x
+    want = [[0, 1, 4, 5, 12], [2, 3, 6, 7, 13], [8, 9, 10, 11, 14]]
+    for y in range(3):
+        for x in range(5):
+            assert O.morton_rank([5, 3], [x, y]) == want[y][x]
+
+
+@pytest.mark.parametrize("shape", [[3, 5], [4, 4], [7, 2], [5, 3, 2], [1, 9], [6, 6, 3]])
+def test_morton_rank_brute_force(shape):
+    # rank = number of in-bounds cells with a smaller interleaved Z key
+    def zkey(c):
+        k = 0
+        for bit in range(8):
+            for d in range(len(c)):
+                k |= ((c[d] >> bit) & 1) << (bit * len(c) + d)
+        return k
+    cells = list(np.ndindex(*shape))
+    keys = sorted(zkey(c) for c in cells)
+    for c in cells:
+        assert O.morton_rank(list(shape), list(c)) == keys.index(zkey(c))
+
+
+def test_window_ends_cover_children():
+    # windows of consecutive parents tile the child Morton range
+    for child in ([6, 5], [4, 4], [7, 3], [3, 3, 5]):
+        parent = [(c + 1) // 2 for c in child]
+        e = O.window_ends(child, parent)
+        assert e == sorted(e) and e[-1] == int(np.prod(child))
+        lens = np.diff([0] + e)
+        assert lens.min() >= 1 and lens.max() <= 2 ** len(child)
+
+
+def test_fold_level2_chain_pinned():
+    # tests/test_lod.c:919-928: level-2 mean fold of a constant v over its
+    # 4-epoch period is v -> v/2 -> 3v/8 -> 11v/32 (shift-by-level fold)
+    dt = np.dtype(np.uint16)
+    v = np.array([3200], dtype=dt)
+    acc = v.copy()
+    for _ in range(3):
+        acc = O.add_shift(acc, v, 2, dt)
+    assert int(acc[0]) == 3200 * 11 // 32
+
+
+def test_reduce_window_textbook():
+    x = np.array([5, 1, 9, 3], dtype=np.uint16)
+    assert O.reduce_window(x, O.MEAN) == 4
+    assert O.reduce_window(x, O.MIN) == 1 and O.reduce_window(x, O.MAX) == 9
+    assert O.reduce_window(x, O.MEDIAN) == 5       # sorted [1,3,5,9], n/2 = 2
+    assert O.reduce_window(x, O.MAX_SUP) == 5 and O.reduce_window(x, O.MIN_SUP) == 3
+    xi = np.array([-3, -4], dtype=np.int16)
+    assert O.reduce_window(xi, O.MEAN) == -3      # C truncation toward zero
+
+
+CASES = [
+    ([Dim(4, 2, 2), Dim(4, 2, 2), Dim(6, 3, 2)], "u16", "mean", "mean", 2 * 48),
+    ([Dim(6, 2, 1), Dim(8, 4, 1), Dim(9, 3, 1)], "u16", "mean", "mean", 432),
+    ([Dim(20, 8, 2), Dim(16, 8, 1), Dim(24, 8, 3)], "u8", "mean", "mean", 20 * 16 * 24),
+    ([Dim(8, 4, 0), Dim(12, 4, 0, storage_position=2), Dim(16, 8, 0, storage_position=1)],
+     "i32", "mean", "mean", 8 * 12 * 16),
+    ([Dim(8, 4, 1), Dim(12, 4, 0, downsample=True), Dim(10, 4, 0, downsample=True)],
+     "u16", "median", "mean", 8 * 12 * 10),
+    ([Dim(16, 2, 4, downsample=True), Dim(8, 4, 0, downsample=True), Dim(12, 4, 0, downsample=True)],
+     "u8", "mean", "mean", 16 * 8 * 12),
+    ([Dim(12, 2, 2, downsample=True), Dim(8, 4, 0, downsample=True), Dim(8, 4, 0, downsample=True)],
+     "f32", "max_sup", "max", 12 * 8 * 8),
+    ([Dim(10, 2, 2, downsample=True), Dim(6, 4, 0, downsample=True), Dim(10, 4, 0, downsample=True)],
+     "i16", "min", "min", 10 * 6 * 10),
+]
+
+
+def _odims(dims):
+    return [O.ODim(d.size, d.chunk, d.cps, d.downsample,
+                   -1 if d.storage_position is None else d.storage_position) for d in dims]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_restatement_matches_reference(case):
+    dims, dtype, red, ared, n = CASES[case]
+    cfg = ca.Config(dims, dtype=dtype, reduce=red, append_reduce=ared, buffer_capacity_bytes=4096)
+    data = random_typed(n, dtype, seed=100 + case)
+    want, _, wcur = run_ref(cfg, data, pieces=3)
+    got, gcur = O.write_stream(_odims(dims), ca.NP_DTYPE[dtype], data,
+                               reduce_method=O.__dict__[{"mean": "MEAN", "min": "MIN", "max": "MAX",
+                                                         "median": "MEDIAN", "max_sup": "MAX_SUP",
+                                                         "min_sup": "MIN_SUP"}[red]],
+                               append_method={"mean": O.MEAN, "min": O.MIN, "max": O.MAX}[ared])
+    assert gcur == wcur
+    assert set(got) == set(want)
+    for k in want:
+        assert got[k] == want[k], f"shard {k}"
+
+
+def _golden_files():
+    if not os.path.isdir(GOLDEN):
+        return []
+    return sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+@pytest.mark.parametrize("name", _golden_files())
+def test_restatement_matches_golden(name):
+    g = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    spec = json.loads(g["spec"].tobytes().decode())
+    dims = [O.ODim(int(a), int(b), int(c), bool(d), int(e)) for a, b, c, d, e in spec["dims"]]
+    data = g["input"]
+    got, _ = O.write_stream(dims, data.dtype, data, reduce_method=int(spec["reduce"]),
+                            append_method=int(spec["append"]))
+    keys = [tuple(int(v) for v in k) for k in g["keys"]]
+    assert sorted(got) == sorted(keys)
+    for i, k in enumerate(keys):
+        assert got[k] == g[f"shard{i}"].tobytes(), f"shard {k}"
+
+
+def test_shard_index_layout_closed_form():
+    # one epoch of a 2x2x2-chunk volume in one shard: index entries are
+    # consecutive chunk-sized runs and the trailer is the CRC of the index
+    dims = [O.ODim(2, 2, 1), O.ODim(4, 2, 0), O.ODim(4, 2, 0)]
+    data = np.arange(32, dtype=np.uint16)
+    out, _ = O.write_stream(dims, np.uint16, data)
+    (k, blob), = out.items()
+    idx = blob[-(16 * 4 + 4):-4]
+    assert struct.unpack("<I", blob[-4:])[0] == O.crc32c(idx)
+    ents = [struct.unpack_from("<QQ", idx, 16 * i) for i in range(4)]
+    assert ents == [(0, 16), (16, 16), (32, 16), (48, 16)]
