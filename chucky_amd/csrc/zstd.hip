@@ -8,17 +8,22 @@
 // each segment at a fixed stride; the aggregate gather concatenates them.
 //
 // Per block (RFC 8878 3.1.1):
-//  * RLE block when every byte is equal;
-//  * greedy wave-parallel LZ77 parse (64 candidate positions per step, LDS
-//    hash table, matches stay inside the block) -> sequences + literals;
+//  * greedy wave-parallel LZ77 parse: 64 candidate positions per step, a
+//    6-byte key hashed into an LDS table, matches of >= 6 bytes inside the
+//    block; after long literal runs the 64 lanes spread out (zstd-style
+//    search acceleration: stride 1 + run/256), so incompressible stretches
+//    cost little; literal runs are compacted to scratch while a single
+//    256-bin LDS histogram counts them;
+//  * RLE block when the parse finds one offset-1 match covering the block;
 //  * literals: Huffman, 4 streams, table built per block (FSE-compressed or
-//    direct weights), streams encoded by the whole wave: per-lane bit
-//    lengths -> wave prefix sum -> LDS atomic-OR into a staging window ->
-//    completed bytes copied out; raw literals when Huffman does not pay;
+//    direct weights); each stream is encoded by the whole wave (per-lane bit
+//    lengths -> wave prefix sum -> LDS atomic-OR staging -> bytes copied out)
+//    straight to its final position; the jump table is filled in after the
+//    streams. Raw literals when Huffman does not pay;
 //  * sequences: predefined FSE distributions;
 //  * raw block whenever the compressed form is not smaller.
-// The work list is either one workgroup per block (plain pipeline) or a
-// persistent grid pulling block indices from a device counter (batched).
+// The work list is a persistent grid pulling block indices from a device
+// counter (one queue per launch).
 #include "kernels.hpp"
 #include "wave.hpp"
 #include "zstd_core.hpp"
@@ -31,6 +36,7 @@ using zcore::Seq;
 constexpr uint32_t kBlock = zcore::kBlockMax;
 constexpr uint64_t kSegStride = kBlock + 64; // >= frame header + block header + raw block
 constexpr int kHashLog = 12;
+constexpr uint32_t kMinMatch = 6;
 constexpr uint32_t kMaxSeq = 16384;
 constexpr uint64_t kLitsBytes = kBlock + 256;
 constexpr uint64_t kSeqBytes = (uint64_t)kMaxSeq * sizeof(Seq);
@@ -46,16 +52,33 @@ struct Lds
     uint32_t table[1 << kHashLog];
     zcore::HufWork huf;
   } u;
-  uint32_t hist[4][256];
-  uint32_t count[256];
+  uint32_t hist[256];
   zcore::HufCode code;
   zcore::Predefined pre;
   uint8_t tree[192];
   uint32_t stage[kStageWords + 4];
   // decisions published by lane 0
-  uint32_t mode, tree_size, lit_size, lit_hsize, comp_lits, seq_size;
-  uint32_t stream_bytes[4];
+  uint32_t mode, tree_size, lit_size, lit_hsize, seq_size;
 };
+
+__device__ __forceinline__ uint32_t key_hash(uint64_t key48)
+{
+  return (uint32_t)((key48 * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
+}
+
+// Compacts in[0, n) to dst and adds every byte to the LDS histogram.
+__device__ __forceinline__ void copy_count(uint8_t* dst, const uint8_t* src, uint32_t n,
+                                           uint32_t* hist)
+{
+  wave_copy(dst, src, n);
+  const int lane = lane_id();
+  for (uint32_t i = 4u * lane; i < n; i += 4u * kWave) {
+    const uint32_t w = ld32u(src + i);
+    const uint32_t m = n - i < 4 ? n - i : 4;
+    for (uint32_t k = 0; k < m; ++k)
+      atomicAdd(&hist[(w >> (8 * k)) & 0xFF], 1u);
+  }
+}
 
 // Encodes lits[a, b) as one backward Huffman stream at out; returns its size.
 __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t a, uint32_t b,
@@ -68,13 +91,16 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
   uint32_t flushed = 0; // bytes already copied out
   uint32_t rel = 0;     // bits held in stage (stage bit 0 = stream bit 8*flushed)
   long top = (long)b - 1;
-  while (top >= (long)a) {
+  for (;;) {
+    // lane L takes symbols top - 4L - 3 .. top - 4L (one dword), highest first
+    const long lo = top - (long)lane * kSymPerLane - (kSymPerLane - 1);
     uint64_t bits = 0;
     uint32_t nb = 0;
-    for (int j = 0; j < kSymPerLane; ++j) {
-      const long idx = top - (long)lane * kSymPerLane - j;
-      if (idx >= (long)a) {
-        const uint8_t s = lits[idx];
+    if (lo + kSymPerLane - 1 >= (long)a) {
+      const uint32_t w = ld32u(lits + (lo >= (long)a ? lo : (long)a));
+      const int skip = lo >= (long)a ? 0 : (int)((long)a - lo);
+      for (int j = kSymPerLane - 1; j >= skip; --j) {
+        const uint8_t s = (uint8_t)(w >> (8 * (j - skip)));
         bits |= (uint64_t)h.val[s] << nb;
         nb += h.len[s];
       }
@@ -96,30 +122,26 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
     __syncthreads();
     rel += total;
     top -= (long)kWave * kSymPerLane;
-    const uint32_t full = rel >> 3;
-    {
-      if (top < (long)a) {
-        // end mark, then everything goes out
-        if (lane == 0)
-          stage[rel >> 5] |= 1u << (rel & 31);
-        __syncthreads();
-        const uint32_t nbytes = (rel + 1 + 7) >> 3;
-        wave_copy(out + flushed, (const uint8_t*)stage, nbytes);
-        __syncthreads();
-        return flushed + nbytes;
-      }
-      wave_copy(out + flushed, (const uint8_t*)stage, full);
+    if (top < (long)a) {
+      if (lane == 0)
+        stage[rel >> 5] |= 1u << (rel & 31); // end mark
       __syncthreads();
-      const uint32_t carry = ((const uint8_t*)stage)[full];
+      const uint32_t nbytes = (rel + 1 + 7) >> 3;
+      wave_copy(out + flushed, (const uint8_t*)stage, nbytes);
       __syncthreads();
-      for (int i = lane; i < kStageWords + 4; i += kWave)
-        stage[i] = (i == 0) ? carry : 0u;
-      __syncthreads();
-      flushed += full;
-      rel &= 7;
+      return flushed + nbytes;
     }
+    const uint32_t full = rel >> 3;
+    wave_copy(out + flushed, (const uint8_t*)stage, full);
+    __syncthreads();
+    const uint32_t carry = ((const uint8_t*)stage)[full];
+    __syncthreads();
+    for (int i = lane; i < kStageWords + 4; i += kWave)
+      stage[i] = (i == 0) ? carry : 0u;
+    __syncthreads();
+    flushed += full;
+    rel &= 7;
   }
-  return flushed; // not reached for a non-empty range
 }
 
 // Compresses one block; returns bytes written at out (header included).
@@ -128,148 +150,124 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
 {
   const int lane = lane_id();
 
-  // ---- RLE block ----
-  {
-    const uint32_t b0 = in[0];
-    bool diff = false;
-    for (uint32_t i = 4u * lane; i < len; i += 4u * kWave) {
-      const uint32_t w = ld32u(in + i);
-      const uint32_t nbytes = len - i < 4 ? len - i : 4;
-      for (uint32_t k = 0; k < nbytes; ++k)
-        diff |= ((w >> (8 * k)) & 0xFF) != b0;
-    }
-    if (wave_ballot(diff) == 0) {
-      if (lane == 0) {
-        zcore::write_block_header(out, last, 1, len);
-        out[3] = (uint8_t)b0;
-      }
-      return 4;
-    }
-  }
-
-  // ---- parse ----
+  // ---- parse + literal compaction + histogram ----
   for (int i = lane; i < (1 << kHashLog); i += kWave)
     L.u.table[i] = 0;
+  for (int i = lane; i < 256; i += kWave)
+    L.hist[i] = 0;
   __syncthreads();
   uint32_t nseq = 0, nlits = 0, anchor = 0;
   if (len >= 16) {
-    const uint32_t last_start = len - 4;
+    const uint32_t last_start = len - kMinMatch; // a key needs 6 readable bytes
     uint32_t pos = 0;
     while (pos <= last_start && nseq < kMaxSeq) {
-      const uint32_t p = pos + (uint32_t)lane;
+      const uint32_t stride = 1 + ((pos - anchor) >> 8);
+      const uint32_t p = pos + (uint32_t)lane * stride;
       const bool active = p <= last_start;
-      uint32_t v = 0, h = 0, cand = 0;
+      uint32_t v0 = 0, v1 = 0, h = 0, cand = 0;
       if (active) {
-        v = ld32u(in + p);
-        h = (v * 2654435761u) >> (32 - kHashLog);
+        v0 = ld32u(in + p);
+        v1 = ld32u(in + p + 4) & 0xFFFFu;
+        h = key_hash(((uint64_t)v1 << 32) | v0);
         cand = L.u.table[h];
       }
       // entries at or after p can be left by lanes past the previous match
-      const bool hit = active && cand != 0 && cand - 1 < p && ld32u(in + cand - 1) == v;
+      const bool hit = active && cand != 0 && cand - 1 < p && ld32u(in + cand - 1) == v0 &&
+                       (ld32u(in + cand + 3) & 0xFFFFu) == v1;
       __builtin_amdgcn_wave_barrier();
       if (active)
         L.u.table[h] = p + 1;
       __builtin_amdgcn_wave_barrier();
       const uint64_t mask = wave_ballot(hit);
       if (mask == 0) {
-        pos += kWave;
+        pos += kWave * stride;
         continue;
       }
       const int first = __ffsll((unsigned long long)mask) - 1;
-      uint32_t mp = pos + (uint32_t)first;
+      uint32_t mp = pos + (uint32_t)first * stride;
       uint32_t mc = wave_bcast(cand, first) - 1;
-      uint32_t ml = wave_match_extend(in, mp, mc, 4, len);
+      uint32_t ml = wave_match_extend(in, mp, mc, kMinMatch, len);
       while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
         --mp;
         --mc;
         ++ml;
       }
       const uint32_t ll = mp - anchor;
-      wave_copy(lits + nlits, in + anchor, ll);
+      copy_count(lits + nlits, in + anchor, ll, L.hist);
       nlits += ll;
       if (lane == 0)
         seqs[nseq] = Seq{ ll, ml, mp - mc };
       ++nseq;
       pos = mp + ml;
       anchor = pos;
-      if (lane == 0 && pos >= 2 && pos - 2 <= last_start)
-        L.u.table[(ld32u(in + pos - 2) * 2654435761u) >> (32 - kHashLog)] = pos - 1;
+      if (lane == 0 && pos >= 2 && pos - 2 <= last_start) {
+        const uint32_t q = pos - 2;
+        L.u.table[key_hash(((uint64_t)(ld32u(in + q + 4) & 0xFFFFu) << 32) | ld32u(in + q))] = q + 1;
+      }
       __builtin_amdgcn_wave_barrier();
     }
   }
-  wave_copy(lits + nlits, in + anchor, len - anchor);
+  // one literal plus one offset-1 match to the end: every byte equals in[0]
+  if (nseq == 1 && anchor == len) {
+    uint32_t rle = 0;
+    if (lane == 0)
+      rle = seqs[0].ll == 1 && seqs[0].off == 1;
+    if (wave_bcast(rle, 0)) {
+      if (lane == 0) {
+        zcore::write_block_header(out, last, 1, len);
+        out[3] = in[0];
+      }
+      return 4;
+    }
+  }
+  copy_count(lits + nlits, in + anchor, len - anchor, L.hist);
   nlits += len - anchor;
   __syncthreads();
 
-  // ---- literals: histogram + code ----
-  const uint32_t seg = (nlits + 3) / 4;
+  // ---- literal mode, Huffman code, sequences section (lane 0) ----
   if (lane == 0) {
+    const uint32_t raw_h = nlits < 32 ? 1 : (nlits < 4096 ? 2 : 3);
     L.mode = 0;
-    L.lit_size = (uint32_t)(nlits < 32 ? 1 : (nlits < 4096 ? 2 : 3)) + nlits;
-  }
-  if (nlits >= 256) {
-    for (int i = lane; i < 4 * 256; i += kWave)
-      (&L.hist[0][0])[i] = 0;
-    __syncthreads();
-    for (uint32_t i = 4u * lane; i < nlits; i += 4u * kWave) {
-      const uint32_t w = *(const uint32_t*)(lits + i);
-      for (uint32_t k = 0; k < 4 && i + k < nlits; ++k)
-        atomicAdd(&L.hist[(i + k) / seg][(w >> (8 * k)) & 0xFF], 1u);
-    }
-    __syncthreads();
-    if (lane == 0) {
-      uint32_t* count = L.count;
+    L.lit_size = raw_h + nlits;
+    if (nlits >= 256) {
       int used = 0;
-      for (int s = 0; s < 256; ++s) {
-        count[s] = L.hist[0][s] + L.hist[1][s] + L.hist[2][s] + L.hist[3][s];
-        used += count[s] != 0;
-      }
+      for (int s = 0; s < 256; ++s)
+        used += L.hist[s] != 0;
       if (used == 1) {
-        L.mode = 1; // RLE literals
-        L.lit_size = (uint32_t)(nlits < 32 ? 1 : (nlits < 4096 ? 2 : 3)) + 1;
+        L.mode = 1;
+        L.lit_size = raw_h + 1;
       } else {
-        zcore::huf_build(L.code, count, L.u.huf);
+        zcore::huf_build(L.code, L.hist, L.u.huf);
         const uint32_t tsz = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
         if (tsz) {
-          uint32_t comp = tsz + 6;
-          bool fits = true;
-          for (int k = 0; k < 4; ++k) {
-            uint64_t bits = 0;
-            for (int s = 0; s < 256; ++s)
-              bits += (uint64_t)L.hist[k][s] * L.code.len[s];
-            const uint32_t bytes = (uint32_t)((bits + 8) >> 3);
-            L.stream_bytes[k] = bytes;
-            fits &= (k == 3) || bytes <= 0xFFFF;
-            comp += bytes;
-          }
-          const uint32_t hs = (uint32_t)zcore::huf_lits_header_size(nlits, comp);
-          if (fits && hs + comp < L.lit_size) {
+          uint64_t bits = 0;
+          for (int s = 0; s < 256; ++s)
+            bits += (uint64_t)L.hist[s] * L.code.len[s];
+          // streams: <= bits + 4 end marks, rounded up per stream
+          const uint64_t comp_bound = tsz + 6 + (bits + 4 * 8 + 7) / 8;
+          // a stream holds <= nlits/4+1 symbols of <= 11 bits: 16-bit jump entries hold
+          const uint32_t hs = (uint32_t)zcore::huf_lits_header_size(nlits, nlits);
+          if (comp_bound < nlits && hs + comp_bound < L.lit_size && (nlits + 3) / 4 * 11 / 8 + 2 < 65536) {
             L.mode = 2;
             L.tree_size = tsz;
-            L.comp_lits = comp;
             L.lit_hsize = hs;
-            L.lit_size = hs + comp;
+            L.lit_size = hs + (uint32_t)comp_bound; // bound; exact size known after encoding
           }
         }
       }
     }
-  }
-  // ---- sequences section into scratch ----
-  if (lane == 0)
     L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, L.pre);
+  }
   __syncthreads();
-  const uint32_t mode = L.mode, lit_size = L.lit_size, seq_size = L.seq_size;
-  const uint32_t body = lit_size + seq_size;
-
-  if (body >= len) {
+  const uint32_t mode = L.mode, seq_size = L.seq_size;
+  if (L.lit_size + seq_size >= len) {
     if (lane == 0)
       zcore::write_block_header(out, last, 0, len);
     wave_copy(out + 3, in, len);
     return 3 + len;
   }
   uint8_t* o = out + 3;
-  if (lane == 0)
-    zcore::write_block_header(out, last, 2, body);
+  uint32_t lit_size = L.lit_size;
   if (mode == 0) {
     uint32_t h = 0;
     if (lane == 0)
@@ -283,25 +281,32 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
     }
   } else {
     const uint32_t hs = L.lit_hsize, ts = L.tree_size;
-    if (lane == 0) {
-      zcore::write_huf_lits_header(o, hs, nlits, L.comp_lits);
+    if (lane == 0)
       for (uint32_t i = 0; i < ts; ++i)
         o[hs + i] = L.tree[i];
-      uint8_t* jt = o + hs + ts;
-      for (int k = 0; k < 3; ++k) {
-        jt[2 * k] = (uint8_t)L.stream_bytes[k];
-        jt[2 * k + 1] = (uint8_t)(L.stream_bytes[k] >> 8);
-      }
-    }
-    uint32_t sp = hs + ts + 6;
+    const uint32_t seg = (nlits + 3) / 4;
+    uint32_t sp = hs + ts + 6, sizes[4];
     for (int k = 0; k < 4; ++k) {
       const uint32_t a = (uint32_t)k * seg;
       const uint32_t b = k == 3 ? nlits : a + seg;
-      huf_stream_wave(o + sp, lits, a, b, L.code, L.stage);
-      sp += L.stream_bytes[k];
+      sizes[k] = huf_stream_wave(o + sp, lits, a, b, L.code, L.stage);
+      sp += sizes[k];
     }
+    const uint32_t comp = sp - hs;
+    if (lane == 0) {
+      zcore::write_huf_lits_header(o, hs, nlits, comp);
+      uint8_t* jt = o + hs + ts;
+      for (int k = 0; k < 3; ++k) {
+        jt[2 * k] = (uint8_t)sizes[k];
+        jt[2 * k + 1] = (uint8_t)(sizes[k] >> 8);
+      }
+    }
+    lit_size = hs + comp;
   }
   wave_copy(o + lit_size, seqout, seq_size);
+  const uint32_t body = lit_size + seq_size;
+  if (lane == 0)
+    zcore::write_block_header(out, last, 2, body);
   return 3 + body;
 }
 
