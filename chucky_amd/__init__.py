@@ -218,5 +218,5 @@ class DeviceBuffer:
 from .api import (Config, GpuStream, MemSink, DiscardSink, last_error,  # noqa: E402
                   crc32c, memory_estimate)
 
-__all__ = ["lib", "DeviceBuffer", "Config", "GpuStream", "MemSink", "DiscardSink", "last_error", "crc32c",
+__all__ = ["lib", "PIPELINE_PLAIN", "PIPELINE_BATCHED", "DeviceBuffer", "Config", "GpuStream", "MemSink", "DiscardSink", "last_error", "crc32c",
            "memory_estimate", "DTYPES", "CODEC_ID", "REDUCE_ID"]

@@ -96,7 +96,9 @@ __global__ __launch_bounds__(64) void lz4_chunk_k(const uint8_t* __restrict__ sr
     const uint32_t match_end = n - kLastLiterals; // and end at <= this
     uint32_t pos = 0;
     while (pos <= last_start) {
-      const uint32_t p = pos + (uint32_t)lane;
+      // search acceleration: lanes spread out over long literal runs
+      const uint32_t stride = 1 + ((pos - anchor) >> 10);
+      const uint32_t p = pos + (uint32_t)lane * stride;
       const bool active = p <= last_start;
       uint32_t v = 0, h = 0, cand = 0;
       if (active) {
@@ -118,11 +120,11 @@ __global__ __launch_bounds__(64) void lz4_chunk_k(const uint8_t* __restrict__ sr
       __builtin_amdgcn_wave_barrier();
       const uint64_t mask = wave_ballot(hit);
       if (mask == 0) {
-        pos += kWave;
+        pos += kWave * stride;
         continue;
       }
       const int first = __ffsll((unsigned long long)mask) - 1;
-      uint32_t mp = pos + (uint32_t)first;
+      uint32_t mp = pos + (uint32_t)first * stride;
       uint32_t mc = wave_bcast(cand, first) - 1;
       uint32_t len = wave_match_extend(in, mp, mc, 4, match_end);
       // backward extension over pending literals

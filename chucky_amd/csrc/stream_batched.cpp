@@ -1,0 +1,393 @@
+// Batched pipeline: every epoch is compressed, aggregated and handed to the
+// sink on its own, as soon as it is complete, with no host wait on the device
+// path.
+//
+//   compute queue:  scatter/LOD(e) -> [graph: compress all chunks of epoch e]
+//   aggregate queue: wait(compress e) -> [graph: per active level, size plan +
+//                    exclusive scan, offsets to pinned memory, gather of the
+//                    compressed bytes straight into a pinned host slot]
+//   host:           delivers epochs in order once their aggregate event has
+//                   completed (sink writes, index entries, shard finalisation)
+//
+// The compress graph of epoch e+1 runs while the gather of epoch e streams
+// over PCIe. Epochs rotate through the K pool slots of the plan and through R
+// pinned host slots; a slot is reused only after the work that read it has
+// completed (events on the device side, delivery + sink fences on the host).
+//
+// Output equals the per-K-batch delivery of the plain pipeline byte for byte
+// when shard_alignment == 0 (src/zarr/shard_delivery.c:104-204 appends each
+// shard's runs in epoch order either way); with page alignment the padding
+// depends on the batch grouping, so such streams use the plain pipeline.
+#include "stream_gpu.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace chucky {
+
+namespace {
+
+#define BCHECK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      set_error(std::string("batched: ") + #expr + ": " + hipGetErrorString(e_));           \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+} // namespace
+
+struct BatchedState
+{
+  hipStream_t st_agg = nullptr;
+  uint32_t R = 0; // host slots
+  struct HostSlot
+  {
+    uint8_t* h_agg = nullptr;
+    std::vector<uint64_t> level_base; // byte offset of each level's bytes
+    std::vector<uint64_t*> h_offsets, h_sizes;
+    std::vector<io_event> fence;
+  };
+  struct PoolSlot
+  {
+    std::vector<uint32_t*> d_gather, d_perm;
+    std::vector<uint64_t*> d_sizes, d_offsets;
+    hipGraphExec_t compress_exec = nullptr;
+    hipEvent_t ev_compressed = nullptr; // compress of the slot's epoch done
+    hipEvent_t ev_gathered = nullptr;   // gather of the slot's epoch done
+  };
+  std::vector<HostSlot> host;
+  std::vector<PoolSlot> pool;
+  // aggregate graphs by (pool slot, host slot, mask)
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t>, hipGraphExec_t> agg_exec;
+  struct Ticket
+  {
+    uint64_t epoch;
+    uint32_t pool_slot, host_slot, mask;
+    hipEvent_t done;
+    std::chrono::steady_clock::time_point t0;
+  };
+  std::deque<Ticket> inflight;
+  uint64_t next_epoch = 0;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+SegmentSource source_of(const tile_stream_gpu* s)
+{
+  SegmentSource src{};
+  if (s->plan.config.codec.id == CODEC_NONE) {
+    const uint64_t piece = 256 * 1024;
+    src.base = s->d_pool;
+    src.chunk_stride = s->chunk_stride_bytes;
+    src.nseg = (uint32_t)std::max<uint64_t>(1, (s->chunk_bytes + piece - 1) / piece);
+    src.seg_stride = piece;
+    src.const_size = piece;
+    src.const_total = s->chunk_bytes;
+  } else {
+    src.base = s->d_comp;
+    src.chunk_stride = s->chunk_slot;
+    src.seg_stride = s->seg_stride;
+    src.nseg = s->nseg;
+    src.seg_sizes = s->d_seg_sizes;
+  }
+  return src;
+}
+
+template<typename T>
+int dev_alloc(tile_stream_gpu* s, T** p, size_t count)
+{
+  void* q = nullptr;
+  BCHECK(hipMalloc(&q, count * sizeof(T) + 256));
+  s->allocations.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+template<typename T>
+int host_alloc(tile_stream_gpu* s, T** p, size_t count)
+{
+  void* q = nullptr;
+  BCHECK(hipHostMalloc(&q, count * sizeof(T) + 256, hipHostMallocDefault));
+  s->host_allocations.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+hipEvent_t take_event(BatchedState* b)
+{
+  if (!b->event_pool.empty()) {
+    hipEvent_t e = b->event_pool.back();
+    b->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+    return nullptr;
+  return e;
+}
+
+int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
+{
+  const auto codec = s->plan.config.codec;
+  const uint64_t T = s->plan.total_chunks;
+  if (codec.id == CODEC_LZ4_NON_STANDARD)
+    BCHECK(launch_lz4(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
+                      s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
+                      s->d_seg_sizes + (uint64_t)slot * T * s->nseg, s->lz4_hash_log, st));
+  else if (codec.id == CODEC_ZSTD)
+    BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
+                       s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
+                       s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->d_zscratch,
+                       s->zworkers, s->d_queue, st));
+  return 0;
+}
+
+int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t hs, uint32_t mask,
+                     hipStream_t st)
+{
+  const SegmentSource src = source_of(s);
+  auto& P = b->pool[ps];
+  auto& H = b->host[hs];
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    if (!(mask & (1u << lv)))
+      continue;
+    const LevelInfo& L = s->plan.levels[lv];
+    const uint64_t M = L.agg.chunks_per_epoch, C = L.agg.covering_count;
+    BCHECK(launch_aggregate_plan(src, P.d_gather[lv], P.d_perm[lv], M, C, L.agg.cps_inner, 0,
+                                 P.d_sizes[lv], P.d_offsets[lv], st));
+    BCHECK(hipMemcpyAsync(H.h_offsets[lv], P.d_offsets[lv], (C + 1) * 8, hipMemcpyDeviceToHost, st));
+    BCHECK(hipMemcpyAsync(H.h_sizes[lv], P.d_sizes[lv], C * 8, hipMemcpyDeviceToHost, st));
+    BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
+                                   H.h_agg + H.level_base[lv], st));
+  }
+  return 0;
+}
+
+// Capture `fn(stream)` into an executable graph.
+template<typename F>
+int capture(hipStream_t st, hipGraphExec_t* exec, F fn)
+{
+  hipGraph_t g = nullptr;
+  BCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = fn(st);
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc || e != hipSuccess) {
+    if (g)
+      (void)hipGraphDestroy(g);
+    set_error("batched: graph capture failed");
+    return 1;
+  }
+  const hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    set_error(std::string("batched: graph instantiate: ") + hipGetErrorString(ie));
+    return 1;
+  }
+  return 0;
+}
+
+// Hands the oldest in-flight epoch to the sink once its gather is done.
+int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
+{
+  if (b->inflight.empty())
+    return 0;
+  auto& t = b->inflight.front();
+  if (block) {
+    BCHECK(hipEventSynchronize(t.done));
+  } else {
+    const hipError_t q = hipEventQuery(t.done);
+    if (q == hipErrorNotReady)
+      return 2;
+    BCHECK(q);
+  }
+  auto& H = b->host[t.host_slot];
+  const auto t1 = std::chrono::steady_clock::now();
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    if (!(t.mask & (1u << lv)))
+      continue;
+    AggView v{ H.h_agg + H.level_base[lv], H.h_offsets[lv], H.h_sizes[lv] };
+    size_t out = 0;
+    if (deliver_batch((uint8_t)lv, s->shards[lv], v, 1, s->sink, 0, &out))
+      return 1;
+    s->metrics.sink.input_bytes += (double)out;
+    s->metrics.d2h.input_bytes += (double)out;
+    if (s->sink->record_fence)
+      H.fence[lv] = s->sink->record_fence(s->sink, (uint8_t)lv);
+  }
+  const auto t2 = std::chrono::steady_clock::now();
+  s->metrics.sink.ms += std::chrono::duration<float, std::milli>(t2 - t1).count();
+  s->metrics.sink.count++;
+  s->latencies.push_back(std::chrono::duration<double>(t2 - t.t0).count());
+  s->epochs_written++;
+  b->event_pool.push_back(t.done);
+  b->inflight.pop_front();
+  return 0;
+}
+
+} // namespace
+
+int batched_init(tile_stream_gpu* s)
+{
+  auto* b = new BatchedState();
+  s->batched = b;
+  const Plan& p = s->plan;
+  const uint32_t K = p.epochs_per_batch;
+  BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
+  b->R = std::min<uint32_t>(K, 4);
+  // worst-case bytes of one epoch per level
+  std::vector<uint64_t> level_bytes(p.nlod);
+  uint64_t total = 0;
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    const uint64_t per = p.config.codec.id == CODEC_NONE ? s->chunk_bytes : p.max_output_size;
+    level_bytes[lv] = align_up(p.levels[lv].agg.chunks_per_epoch * per, 256);
+    total += level_bytes[lv];
+  }
+  b->host.resize(b->R);
+  for (auto& H : b->host) {
+    if (host_alloc(s, &H.h_agg, total))
+      return 1;
+    H.level_base.assign(p.nlod, 0);
+    H.h_offsets.assign(p.nlod, nullptr);
+    H.h_sizes.assign(p.nlod, nullptr);
+    H.fence.assign(p.nlod, io_event{ 0 });
+    uint64_t base = 0;
+    for (int lv = 0; lv < p.nlod; ++lv) {
+      H.level_base[lv] = base;
+      base += level_bytes[lv];
+      const uint64_t C = p.levels[lv].agg.covering_count;
+      if (host_alloc(s, &H.h_offsets[lv], C + 1) || host_alloc(s, &H.h_sizes[lv], C))
+        return 1;
+    }
+  }
+  b->pool.resize(K);
+  std::vector<uint32_t> g, pm;
+  for (uint32_t ps = 0; ps < K; ++ps) {
+    auto& P = b->pool[ps];
+    P.d_gather.assign(p.nlod, nullptr);
+    P.d_perm.assign(p.nlod, nullptr);
+    P.d_sizes.assign(p.nlod, nullptr);
+    P.d_offsets.assign(p.nlod, nullptr);
+    for (int lv = 0; lv < p.nlod; ++lv) {
+      const uint64_t M = p.levels[lv].agg.chunks_per_epoch, C = p.levels[lv].agg.covering_count;
+      if (dev_alloc(s, &P.d_gather[lv], M) || dev_alloc(s, &P.d_perm[lv], M) ||
+          dev_alloc(s, &P.d_sizes[lv], C) || dev_alloc(s, &P.d_offsets[lv], C + 1))
+        return 1;
+      g.assign(M, 0);
+      pm.assign(M, 0);
+      const uint32_t pe[1] = { ps };
+      batch_luts(p, lv, 1, pe, g.data(), pm.data());
+      BCHECK(hipMemcpy(P.d_gather[lv], g.data(), M * 4, hipMemcpyHostToDevice));
+      BCHECK(hipMemcpy(P.d_perm[lv], pm.data(), M * 4, hipMemcpyHostToDevice));
+    }
+    BCHECK(hipEventCreateWithFlags(&P.ev_compressed, hipEventDisableTiming));
+    BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
+    BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
+    if (p.config.codec.id != CODEC_NONE &&
+        capture(s->st_comp, &P.compress_exec, [&](hipStream_t st) { return launch_compress(s, ps, st); }))
+      return 1;
+  }
+  return 0;
+}
+
+void batched_free(tile_stream_gpu* s)
+{
+  BatchedState* b = s->batched;
+  if (!b)
+    return;
+  if (b->st_agg)
+    (void)hipStreamSynchronize(b->st_agg);
+  for (auto& P : b->pool) {
+    if (P.compress_exec)
+      (void)hipGraphExecDestroy(P.compress_exec);
+    if (P.ev_compressed)
+      (void)hipEventDestroy(P.ev_compressed);
+    if (P.ev_gathered)
+      (void)hipEventDestroy(P.ev_gathered);
+  }
+  for (auto& kv : b->agg_exec)
+    (void)hipGraphExecDestroy(kv.second);
+  for (auto& t : b->inflight)
+    (void)hipEventDestroy(t.done);
+  for (auto e : b->event_pool)
+    (void)hipEventDestroy(e);
+  if (b->st_agg)
+    (void)hipStreamDestroy(b->st_agg);
+  delete b;
+  s->batched = nullptr;
+}
+
+// Epoch `pool_slot` of the pool is complete (scatter / LOD already queued on
+// the compute queue): compress, aggregate and queue it for delivery.
+int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
+{
+  BatchedState* b = s->batched;
+  const uint64_t ticket = b->next_epoch++;
+  const uint32_t hs = (uint32_t)(ticket % b->R);
+  // the host slot's previous epoch must be delivered and its writes retired
+  while (!b->inflight.empty() && b->inflight.front().epoch + b->R <= ticket)
+    if (deliver_one(s, b, true))
+      return 1;
+  auto& H = b->host[hs];
+  if (s->sink->wait_fence)
+    for (int lv = 0; lv < s->plan.nlod; ++lv)
+      s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
+  auto& P = b->pool[pool_slot];
+  // compressed slots of this pool slot are free once its last gather is done
+  BCHECK(hipStreamWaitEvent(s->st_comp, P.ev_gathered, 0));
+  if (P.compress_exec)
+    BCHECK(hipGraphLaunch(P.compress_exec, s->st_comp));
+  BCHECK(hipEventRecord(P.ev_compressed, s->st_comp));
+  BCHECK(hipStreamWaitEvent(b->st_agg, P.ev_compressed, 0));
+  const auto key = std::make_tuple(pool_slot, hs, mask);
+  auto it = b->agg_exec.find(key);
+  if (it == b->agg_exec.end()) {
+    hipGraphExec_t ex = nullptr;
+    if (capture(b->st_agg, &ex, [&](hipStream_t st) { return launch_aggregate(s, b, pool_slot, hs, mask, st); }))
+      return 1;
+    it = b->agg_exec.emplace(key, ex).first;
+  }
+  BCHECK(hipGraphLaunch(it->second, b->st_agg));
+  BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
+  hipEvent_t done = take_event(b);
+  if (!done) {
+    set_error("batched: event creation failed");
+    return 1;
+  }
+  BCHECK(hipEventRecord(done, b->st_agg));
+  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, std::chrono::steady_clock::now() });
+  // opportunistic in-order delivery of whatever has finished
+  for (;;) {
+    const int r = deliver_one(s, b, false);
+    if (r == 1)
+      return 1;
+    if (r == 2 || b->inflight.empty())
+      break;
+  }
+  return 0;
+}
+
+int batched_drain(tile_stream_gpu* s)
+{
+  BatchedState* b = s->batched;
+  while (!b->inflight.empty())
+    if (deliver_one(s, b, true))
+      return 1;
+  if (s->sink->wait_fence)
+    for (auto& H : b->host)
+      for (int lv = 0; lv < s->plan.nlod; ++lv)
+        s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
+  BCHECK(hipStreamSynchronize(s->st_comp));
+  BCHECK(hipStreamSynchronize(b->st_agg));
+  return 0;
+}
+
+void batched_reset(tile_stream_gpu* s)
+{
+  if (s->batched)
+    s->batched->next_epoch = 0;
+}
+
+} // namespace chucky

@@ -363,14 +363,20 @@ int end_epoch(tile_stream_gpu* s)
       return 1;
     HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
   }
-  if (s->batch_acc >= MAX_BATCH_EPOCHS) {
+  if (s->batched) {
+    s->epochs_done++;
+    if (batched_submit(s, s->batch_acc, mask))
+      return 1;
+    s->batch_acc = (s->batch_acc + 1) % s->plan.epochs_per_batch;
+  } else if (s->batch_acc >= MAX_BATCH_EPOCHS) {
     set_error("batch overflow");
     return 1;
+  } else {
+    s->masks[s->batch_acc++] = mask;
+    s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
+    s->epochs_done++;
   }
-  s->masks[s->batch_acc++] = mask;
-  s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
-  s->epochs_done++;
-  if (s->batch_acc == s->plan.epochs_per_batch) {
+  if (!s->batched && s->batch_acc == s->plan.epochs_per_batch) {
     if (flush_batch(s, s->batch_acc, s->masks))
       return 1;
     s->batch_acc = 0;
@@ -395,13 +401,24 @@ int flush_all(tile_stream_gpu* s)
     uint32_t mask = 1;
     if (s->plan.enable_multiscale && lod_run_epoch(s, s->batch_acc, &mask))
       return 1;
-    if (s->batch_acc >= MAX_BATCH_EPOCHS)
-      return 1;
-    s->masks[s->batch_acc++] = mask;
-    s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
-    s->epochs_done++;
+    if (s->batched) {
+      s->epochs_done++;
+      if (batched_submit(s, s->batch_acc, mask))
+        return 1;
+      s->batch_acc = (s->batch_acc + 1) % s->plan.epochs_per_batch;
+    } else {
+      if (s->batch_acc >= MAX_BATCH_EPOCHS)
+        return 1;
+      s->masks[s->batch_acc++] = mask;
+      s->epoch_t0[s->epochs_done % s->epoch_t0.size()] = std::chrono::steady_clock::now();
+      s->epochs_done++;
+    }
   }
-  if (s->batch_acc > 0) {
+  if (s->batched) {
+    if (batched_drain(s))
+      return 1;
+    s->batch_acc = 0;
+  } else if (s->batch_acc > 0) {
     if (flush_batch(s, s->batch_acc, s->masks))
       return 1;
     s->batch_acc = 0;
@@ -518,6 +535,7 @@ void free_stream(tile_stream_gpu* s)
   if (!s)
     return;
   (void)hipSetDevice(s->device);
+  batched_free(s);
   if (s->st_comp)
     (void)hipStreamSynchronize(s->st_comp);
   if (s->st_h2d)
@@ -684,6 +702,10 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   s->metrics.lod_morton_chunk = mk_metric("lod_morton");
   s->meta_clock = std::chrono::steady_clock::now();
   s->epoch_t0.resize(MAX_BATCH_EPOCHS * 2);
+  // The batched pipeline delivers per epoch, which equals per-batch delivery
+  // only without shard page alignment; aligned streams stay on the plain path.
+  if (s->pipeline == CHUCKY_PIPELINE_BATCHED && config->shard_alignment == 0 && batched_init(s))
+    return 1;
   HIPCHECK(hipStreamSynchronize(s->st_comp));
   return 0;
 }
@@ -776,6 +798,7 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   std::fill(s->io_done.begin(), s->io_done.end(), io_event{ 0 });
   s->latencies.clear();
   s->epochs_done = s->epochs_written = 0;
+  batched_reset(s);
   return 0;
 }
 

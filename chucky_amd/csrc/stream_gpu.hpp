@@ -9,8 +9,15 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <deque>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
+
+namespace chucky {
+struct BatchedState;
+}
 
 struct tile_stream_gpu
 {
@@ -99,6 +106,8 @@ struct tile_stream_gpu
   };
   std::vector<PendingTiming> pending_scatter;
 
+  chucky::BatchedState* batched = nullptr; // non-null: batched pipeline active
+
   std::vector<void*> allocations;      // device
   std::vector<void*> host_allocations; // pinned host
 };
@@ -108,4 +117,9 @@ void set_error(const std::string& msg);
 int lod_run_epoch(tile_stream_gpu* s, uint32_t epoch_in_batch, uint32_t* out_mask);
 int lod_drain(tile_stream_gpu* s, uint32_t* out_mask);
 int lod_init(tile_stream_gpu* s);
+int batched_init(tile_stream_gpu* s);
+void batched_free(tile_stream_gpu* s);
+int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);
+int batched_drain(tile_stream_gpu* s);
+void batched_reset(tile_stream_gpu* s);
 } // namespace chucky

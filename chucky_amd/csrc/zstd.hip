@@ -66,6 +66,27 @@ __device__ __forceinline__ uint32_t key_hash(uint64_t key48)
   return (uint32_t)((key48 * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
 }
 
+// Ascending bitonic sort of 256 keys in LDS by one wavefront (two
+// compare-exchange pairs per lane per stage).
+__device__ void wave_sort256(uint32_t* keys)
+{
+  const uint32_t lane = (uint32_t)lane_id();
+  for (uint32_t k = 2; k <= 256; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = lane; t < 128; t += kWave) {
+        const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        const uint32_t l = i | j;
+        const bool asc = (i & k) == 0;
+        const uint32_t a = keys[i], b = keys[l];
+        if ((a > b) == asc) {
+          keys[i] = b;
+          keys[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+}
+
 // Compacts in[0, n) to dst and adds every byte to the LDS histogram.
 __device__ __forceinline__ void copy_count(uint8_t* dst, const uint8_t* src, uint32_t n,
                                            uint32_t* hist)
@@ -224,40 +245,55 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
   nlits += len - anchor;
   __syncthreads();
 
-  // ---- literal mode, Huffman code, sequences section (lane 0) ----
+  // ---- literal mode and Huffman code ----
+  const uint32_t raw_h = nlits < 32 ? 1 : (nlits < 4096 ? 2 : 3);
+  uint32_t used_local = 0;
+  for (int s = lane; s < 256; s += kWave)
+    used_local += L.hist[s] != 0;
+  const uint32_t used = wave_sum(used_local);
   if (lane == 0) {
-    const uint32_t raw_h = nlits < 32 ? 1 : (nlits < 4096 ? 2 : 3);
     L.mode = 0;
     L.lit_size = raw_h + nlits;
-    if (nlits >= 256) {
-      int used = 0;
-      for (int s = 0; s < 256; ++s)
-        used += L.hist[s] != 0;
-      if (used == 1) {
-        L.mode = 1;
-        L.lit_size = raw_h + 1;
-      } else {
-        zcore::huf_build(L.code, L.hist, L.u.huf);
-        const uint32_t tsz = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
-        if (tsz) {
-          uint64_t bits = 0;
-          for (int s = 0; s < 256; ++s)
-            bits += (uint64_t)L.hist[s] * L.code.len[s];
-          // streams: <= bits + 4 end marks, rounded up per stream
-          const uint64_t comp_bound = tsz + 6 + (bits + 4 * 8 + 7) / 8;
-          // a stream holds <= nlits/4+1 symbols of <= 11 bits: 16-bit jump entries hold
-          const uint32_t hs = (uint32_t)zcore::huf_lits_header_size(nlits, nlits);
-          if (comp_bound < nlits && hs + comp_bound < L.lit_size && (nlits + 3) / 4 * 11 / 8 + 2 < 65536) {
-            L.mode = 2;
-            L.tree_size = tsz;
-            L.lit_hsize = hs;
-            L.lit_size = hs + (uint32_t)comp_bound; // bound; exact size known after encoding
-          }
-        }
+    L.tree_size = 0;
+    if (nlits >= 256 && used == 1) {
+      L.mode = 1;
+      L.lit_size = raw_h + 1;
+    }
+  }
+  if (nlits >= 256 && used > 1) {
+    // used symbols in ascending count order: bitonic sort of (count, symbol)
+    uint32_t* keys = L.u.huf.w;
+    for (int s = lane; s < 256; s += kWave)
+      keys[s] = (L.hist[s] << 8) | (uint32_t)s;
+    __syncthreads();
+    wave_sort256(keys);
+    for (uint32_t i = lane; i < used; i += kWave)
+      L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
+    __syncthreads();
+    if (lane == 0) {
+      zcore::huf_build_sorted(L.code, L.hist, (int)used, L.u.huf);
+      L.tree_size = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
+    }
+    __syncthreads();
+    uint32_t bits_local = 0;
+    for (int s = lane; s < 256; s += kWave)
+      bits_local += L.hist[s] * L.code.len[s];
+    const uint32_t bits = wave_sum(bits_local);
+    if (lane == 0 && L.tree_size) {
+      // streams: <= bits + 4 end marks, rounded up per stream; a stream holds
+      // <= nlits/4+1 symbols of <= 11 bits, so 16-bit jump entries suffice
+      const uint64_t comp_bound = L.tree_size + 6 + (bits + 4 * 8 + 7) / 8;
+      const uint32_t hs = (uint32_t)zcore::huf_lits_header_size(nlits, nlits);
+      if (comp_bound < nlits && hs + comp_bound < L.lit_size) {
+        L.mode = 2;
+        L.lit_hsize = hs;
+        L.lit_size = hs + (uint32_t)comp_bound; // bound; exact size known after encoding
       }
     }
-    L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, L.pre);
   }
+  // ---- sequences section into scratch (lane 0) ----
+  if (lane == 0)
+    L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, L.pre);
   __syncthreads();
   const uint32_t mode = L.mode, seq_size = L.seq_size;
   if (L.lit_size + seq_size >= len) {

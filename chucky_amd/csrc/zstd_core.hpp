@@ -463,20 +463,17 @@ struct HufWork
 
 // Code lengths <= kHufMaxBits forming a complete prefix code. count has 256
 // entries; needs >= 2 used symbols.
+// Same as huf_build, with ws.sym[0..n) already holding the used symbols in
+// ascending count order (the GPU kernel sorts them with the whole wave).
+ZHD inline void huf_build_sorted(HufCode& h, const uint32_t* count, int n, HufWork& ws);
+
 ZHD inline void huf_build(HufCode& h, const uint32_t* count, HufWork& ws)
 {
   uint16_t* sym = ws.sym;
-  uint32_t* w = ws.w;
-  int16_t* parent = ws.parent;
   int n = 0;
-  h.max_sym = 0;
-  for (int s = 0; s < 256; ++s) {
-    h.len[s] = 0;
-    if (count[s]) {
+  for (int s = 0; s < 256; ++s)
+    if (count[s])
       sym[n++] = (uint16_t)s;
-      h.max_sym = s;
-    }
-  }
   // ascending by count (stable insertion sort; n <= 256)
   for (int i = 1; i < n; ++i) {
     const uint16_t k = sym[i];
@@ -487,6 +484,19 @@ ZHD inline void huf_build(HufCode& h, const uint32_t* count, HufWork& ws)
     }
     sym[j] = k;
   }
+  huf_build_sorted(h, count, n, ws);
+}
+
+ZHD inline void huf_build_sorted(HufCode& h, const uint32_t* count, int n, HufWork& ws)
+{
+  uint16_t* sym = ws.sym;
+  uint32_t* w = ws.w;
+  int16_t* parent = ws.parent;
+  h.max_sym = 0;
+  for (int s = 0; s < 256; ++s)
+    h.len[s] = 0;
+  for (int i = 0; i < n; ++i)
+    h.max_sym = sym[i] > h.max_sym ? sym[i] : h.max_sym;
   for (int i = 0; i < n; ++i)
     w[i] = count[sym[i]];
   // two-queue Huffman: leaves 0..n-1, internal nodes n..2n-2

@@ -37,10 +37,14 @@ static size_t block(const uint8_t* in, uint32_t len, uint8_t* out, bool last,
       const uint32_t h = (v * 2654435761u) >> 20;
       const uint32_t c = table[h];
       table[h] = p + 1;
+      uint32_t ml = 0;
       if (c && rd32(in + c - 1) == v) {
-        uint32_t mc = c - 1, mp = p, ml = 4;
-        while (mp + ml < len && in[mp + ml] == in[mc + ml])
+        ml = 4;
+        while (p + ml < len && in[p + ml] == in[c - 1 + ml])
           ++ml;
+      }
+      if (ml >= 6) { // minimum match of the GPU parser
+        uint32_t mc = c - 1, mp = p;
         while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
           --mp;
           --mc;

@@ -143,6 +143,31 @@ def test_multiscale_dim0(dtype, append_reduce):
     _compare(cfg, data, pieces=4)
 
 
+BATCHED_CASES = {
+    "volume_zstd": (lambda: xor_volume((64, 64, 64)),
+                    [Dim(64, 16, 2), Dim(64, 16, 2), Dim(64, 16, 4)], "u16", "zstd", {}),
+    "volume_lz4_k2": (lambda: xor_volume((64, 64, 64)),
+                      [Dim(64, 16, 2), Dim(64, 16, 2), Dim(64, 16, 4)], "u16", "lz4",
+                      {"epochs_per_batch": 2}),
+    "partial_none": (lambda: random_typed(20 * 16 * 24, "u8", seed=11),
+                     [Dim(20, 8, 2), Dim(16, 8, 1), Dim(24, 8, 3)], "u8", "none", {}),
+    "lod_inner": (lambda: random_typed(16 * 48 * 40, "u16", seed=21),
+                  [Dim(16, 8, 1), Dim(48, 8, 0, downsample=True), Dim(40, 8, 0, downsample=True)],
+                  "u16", "zstd", {}),
+    "lod_dim0": (lambda: random_typed(40 * 24 * 20, "f32", seed=33),
+                 [Dim(40, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
+                  Dim(20, 4, 0, downsample=True)], "f32", "none", {"epochs_per_batch": 4}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BATCHED_CASES))
+@pytest.mark.parametrize("device_input", [False, True])
+def test_batched_pipeline_matches_reference(name, device_input):
+    make, dims, dtype, codec, extra = BATCHED_CASES[name]
+    cfg = ca.Config(dims, dtype=dtype, codec=codec, **extra)
+    _compare(cfg, make(), pieces=3, device_input=device_input, pipeline=ca.PIPELINE_BATCHED)
+
+
 def test_multiscale_dim0_zstd():
     v = xor_volume((64, 64, 64))
     cfg = ca.Config([Dim(64, 16, 2, downsample=True), Dim(64, 16, 0, downsample=True),
