@@ -181,6 +181,23 @@ _sig("chucky_synth_sensor_frames", C.c_int, _P, C.c_uint64, C.c_uint32, C.c_uint
      C.c_double, C.POINTER(C.c_float), C.c_uint32)
 
 
+class RequestReport(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("end_s", C.c_double), ("input_bytes", C.c_uint64),
+                ("epochs", C.c_uint64), ("p50_latency_s", C.c_double),
+                ("p99_latency_s", C.c_double), ("max_latency_s", C.c_double)]
+
+
+_sig("chucky_server_create", _P, C.c_size_t, C.c_int, C.c_int)
+_sig("chucky_server_submit", C.c_int, _P, C.POINTER(TileStreamConfiguration), _P, _P, C.c_size_t,
+     C.c_int, C.POINTER(C.c_uint64))
+_sig("chucky_server_wait_all", C.c_int, _P)
+_sig("chucky_server_reports", C.c_size_t, _P, C.POINTER(RequestReport), C.c_size_t)
+_sig("chucky_server_latencies", C.c_size_t, _P, C.POINTER(C.c_double), C.c_size_t)
+_sig("chucky_server_peaks", None, _P, C.POINTER(C.c_int), C.POINTER(C.c_size_t))
+_sig("chucky_server_destroy", None, _P)
+
+
 class DeviceBuffer:
     """Device allocation owned by Python (hipMalloc through the library)."""
 

@@ -166,6 +166,57 @@ def memory_estimate(cfg: Config) -> MemoryInfo:
     return info
 
 
+class Server:
+    """chucky_server_*: concurrent streams admitted against an HBM budget."""
+
+    def __init__(self, hbm_budget_bytes: int, max_concurrent: int, pipeline: int = 1):
+        from . import RequestReport  # noqa: F401
+        self.h = lib.chucky_server_create(hbm_budget_bytes, max_concurrent, pipeline)
+        self._keep = []
+
+    def submit(self, cfg: Config, sink, data, device: bool = False) -> int:
+        c = cfg.to_c()
+        rid = C.c_uint64()
+        if device:
+            ptr, nbytes = data.ptr, data.nbytes
+        else:
+            arr = np.ascontiguousarray(data)
+            ptr, nbytes = arr.ctypes.data, arr.nbytes
+            data = arr
+        self._keep.append((cfg, c, sink, data))
+        if lib.chucky_server_submit(self.h, C.byref(c), sink.shard_sink, ptr, nbytes, int(device),
+                                    C.byref(rid)):
+            raise RuntimeError("submit failed: " + last_error())
+        return rid.value
+
+    def wait(self) -> int:
+        return lib.chucky_server_wait_all(self.h)
+
+    def reports(self):
+        from . import RequestReport
+        buf = (RequestReport * 4096)()
+        n = lib.chucky_server_reports(self.h, buf, 4096)
+        return [buf[i] for i in range(n)]
+
+    def latencies(self) -> List[float]:
+        buf = (C.c_double * (1 << 20))()
+        n = lib.chucky_server_latencies(self.h, buf, 1 << 20)
+        return list(buf[:n])
+
+    def peaks(self):
+        r, b = C.c_int(), C.c_size_t()
+        lib.chucky_server_peaks(self.h, C.byref(r), C.byref(b))
+        return r.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.chucky_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class GpuStream:
     """tile_stream_gpu_* through its writer vtable."""
 

@@ -63,6 +63,48 @@ int tile_stream_gpu_level_geometry(const struct tile_stream_gpu* s, int level,
 /* Last error message of this thread ("" if none). */
 const char* chucky_last_error(void);
 
+/* ---- serving loop: many concurrent streams on one GPU ---------------------
+ * Requests (one acquisition each: a configuration, a sink and its input) are
+ * admitted in arrival order while the number of running requests is below
+ * max_concurrent and the sum of their device-memory estimates stays within
+ * hbm_budget_bytes. Each admitted request runs on its own host thread and HIP
+ * queues with the batched pipeline, so work of different requests overlaps on
+ * the device; its partial-epoch and LOD state stay resident until it
+ * finishes. Input is appended epoch by epoch (from host or device memory) to
+ * model a live acquisition. */
+
+struct chucky_server;
+
+struct chucky_request_report
+{
+  uint64_t id;
+  int status;               /* 0 ok, non-zero failure */
+  double submit_s;          /* submit time, seconds since server creation */
+  double start_s;           /* admission time */
+  double end_s;             /* last shard written */
+  uint64_t input_bytes;
+  uint64_t epochs;
+  double p50_latency_s;     /* append -> shard written, per epoch */
+  double p99_latency_s;
+  double max_latency_s;
+};
+
+struct chucky_server* chucky_server_create(size_t hbm_budget_bytes, int max_concurrent, int pipeline);
+/* config, sink and data are borrowed until the request's report is out.
+ * data_is_device: the input lives in device memory. 0 on success. */
+int chucky_server_submit(struct chucky_server* srv, const struct tile_stream_configuration* config,
+                         struct shard_sink* sink, const void* data, size_t nbytes,
+                         int data_is_device, uint64_t* request_id);
+/* Blocks until every submitted request finished. 0 if all succeeded. */
+int chucky_server_wait_all(struct chucky_server* srv);
+/* Copies reports of finished requests (in completion order); returns count. */
+size_t chucky_server_reports(struct chucky_server* srv, struct chucky_request_report* out, size_t cap);
+/* All per-epoch latency samples of finished requests (seconds). */
+size_t chucky_server_latencies(struct chucky_server* srv, double* out, size_t cap);
+/* Peak concurrently running requests and peak admitted device bytes. */
+void chucky_server_peaks(struct chucky_server* srv, int* running, size_t* device_bytes);
+void chucky_server_destroy(struct chucky_server* srv);
+
 /* ---- in-memory and discard sinks ----------------------------------------- */
 
 struct chucky_mem_sink;
