@@ -19,6 +19,13 @@ chunks, 4^3 chunks per shard, lz4 (the reference's CPU-runnable case);
 `multiscale` = the same volume with a mean 2x2 LOD over y, x and dim-0
 downsampling (3 levels), zstd level 1. `--host-input` measures the
 PCIe-inclusive rate through the pinned staging ring instead.
+
+`serving`: 24 requests arriving as a seeded Poisson process (mean 4/s), each
+drawn uniformly from u8 2048 frames of 512x512, u16 256 frames of 2048x2048
+and f32 256^3, with 64^3 or 64x256x256 chunks, zstd or lz4 and 1-5 LOD levels
+over y, x; admitted by the serving loop against a 200 GB HBM budget. value =
+total input GB / (last request done - first arrival); reported beside the
+same requests run one after another through the plain pipeline.
 """
 from __future__ import annotations
 
@@ -162,13 +169,114 @@ def summarize(res, steps):
                        p50_epoch_latency_ms=pct(0.50), p99_epoch_latency_ms=pct(0.99))
 
 
+SERVING_SHAPES = [("u8", (2048, 512, 512)), ("u16", (256, 2048, 2048)), ("f32", (256, 256, 256))]
+
+
+def serving_requests(n=24, rate=4.0, seed=20260404):
+    """Seeded request mix: (arrival_s, dtype, shape, chunk, codec, nlod)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    t -= t[0]
+    out = []
+    for i in range(n):
+        dt, shape = SERVING_SHAPES[rng.integers(3)]
+        chunk = (64, 64, 64) if rng.integers(2) == 0 else (64, 256, 256)
+        codec = "zstd" if rng.integers(2) == 0 else "lz4"
+        nlod = int(rng.integers(1, 6))
+        out.append((float(t[i]), dt, shape, chunk, codec, nlod))
+    return out
+
+
+def serving_config(dt, shape, chunk, codec, nlod):
+    import chucky_amd as ca
+    from chucky_amd.api import Dim
+    lod = nlod > 1
+    dims = [Dim(shape[0], chunk[0], 1, name="t"),
+            Dim(shape[1], min(chunk[1], shape[1]), 4, name="y", downsample=lod),
+            Dim(shape[2], min(chunk[2], shape[2]), 4, name="x", downsample=lod)]
+    cfg = ca.Config(dims, dtype=dt, codec=codec, level=1, max_nlod=nlod if lod else 0,
+                    buffer_capacity_bytes=64 << 20)
+    # a bounded stream never needs a batch longer than itself
+    epochs = -(-shape[0] // chunk[0])
+    k = ca.api.memory_estimate(cfg).epochs_per_batch
+    if k > epochs:
+        cfg.epochs_per_batch = 1 << (epochs - 1).bit_length()
+    return cfg
+
+
+def serving(args):
+    import chucky_amd as ca
+    from chucky_amd.api import Server
+    reqs = serving_requests()
+    inputs = {}
+    for k, (dt, shape) in enumerate(SERVING_SHAPES):
+        n = int(np.prod(shape))
+        buf = ca.DeviceBuffer(n * np.dtype(ca.NP_DTYPE[dt]).itemsize)
+        if ca.lib.chucky_synth_fill(buf.ptr, n, ca.DTYPE_ID[dt], 1000 + k, 0):
+            raise RuntimeError("fill failed: " + ca.last_error())
+        inputs[dt] = buf
+    total = sum(inputs[r[1]].nbytes for r in reqs)
+
+    # serial, plain pipeline: one stream at a time
+    t0 = time.perf_counter()
+    serial_lat = []
+    for _, dt, shape, chunk, codec, nlod in reqs:
+        sink = ca.DiscardSink()
+        s = ca.GpuStream(serving_config(dt, shape, chunk, codec, nlod), sink, pipeline=ca.PIPELINE_PLAIN)
+        eb = s.epoch_elements * np.dtype(ca.NP_DTYPE[dt]).itemsize
+        buf = inputs[dt]
+        for off in range(0, buf.nbytes, eb):
+            assert s.append_device(buf.ptr + off, min(eb, buf.nbytes - off)) == 0, ca.last_error()
+        assert s.flush() == 0, ca.last_error()
+        serial_lat.extend(s.latencies())
+        s.close()
+    serial_s = time.perf_counter() - t0
+
+    srv = Server(hbm_budget_bytes=200 * 10**9, max_concurrent=24, pipeline=ca.PIPELINE_BATCHED)
+    sinks = []
+    t0 = time.perf_counter()
+    for arrival, dt, shape, chunk, codec, nlod in reqs:
+        wait = arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        sinks.append(ca.DiscardSink())
+        srv.submit(serving_config(dt, shape, chunk, codec, nlod), sinks[-1], inputs[dt], device=True)
+    assert srv.wait() == 0, ca.last_error()
+    reps = srv.reports()
+    lat = sorted(srv.latencies())
+    running, peak_bytes = srv.peaks()
+    makespan = max(r.end_s for r in reps) - min(r.submit_s for r in reps)
+    srv.close()
+    pct = lambda v, q: round(v[min(len(v) - 1, int(q * len(v)))] * 1e3, 3) if v else None
+    serial_lat.sort()
+    value = total / makespan / 1e9
+    return {
+        "metric": "end-to-end ingest GB/s (batched vs plain, bit-exact) + p99 shard latency",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(makespan * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8/u16/f32", "data": "synthetic, generated in HBM",
+        "config": {"workload": "serving: 24 Poisson arrivals (4/s) over u8 2048x512x512, "
+                               "u16 256x2048x2048, f32 256^3; 64^3 or 64x256x256 chunks; zstd/lz4; "
+                               "1-5 LOD levels; 200 GB HBM budget; discard sinks",
+                   "input_GB": round(total / 1e9, 3)},
+        "p50_epoch_latency_ms": pct(lat, 0.50), "p99_epoch_latency_ms": pct(lat, 0.99),
+        "peak_concurrent": running, "peak_admitted_GB": round(peak_bytes / 1e9, 3),
+        "serial_plain": {"GBps": round(total / serial_s / 1e9, 3), "seconds": round(serial_s, 3),
+                         "p50_epoch_latency_ms": pct(serial_lat, 0.50),
+                         "p99_epoch_latency_ms": pct(serial_lat, 0.99)},
+        "served_over_serial": round(serial_s / makespan, 3),
+        "arrival_span_s": round(reqs[-1][0], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=512)
-    ap.add_argument("--workload", choices=["sensor", "volume", "multiscale"], default="sensor")
+    ap.add_argument("--workload", choices=["sensor", "volume", "multiscale", "serving"],
+                    default="sensor")
     ap.add_argument("--pipeline", choices=["plain", "batched", "both"], default="both")
     ap.add_argument("--host-input", action="store_true")
     ap.add_argument("--cpu-sample-frames", type=int, default=128)
@@ -178,6 +286,10 @@ def main():
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "serving":
+        print(json.dumps(serving(args)))
+        return
 
     runs = {}
     order = ["plain", "batched"] if args.pipeline == "both" else [args.pipeline]

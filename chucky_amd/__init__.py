@@ -179,6 +179,7 @@ _sig("chucky_memcpy_to_host", C.c_int, _P, _P, C.c_size_t)
 _sig("chucky_device_synchronize", C.c_int)
 _sig("chucky_synth_sensor_frames", C.c_int, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
      C.c_double, C.POINTER(C.c_float), C.c_uint32)
+_sig("chucky_synth_fill", C.c_int, _P, C.c_uint64, C.c_int, C.c_uint64, C.c_uint64)
 
 
 class RequestReport(C.Structure):

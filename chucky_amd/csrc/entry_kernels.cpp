@@ -17,7 +17,7 @@ size_t chucky_gpu_max_output_size(enum compression_codec codec, size_t chunk_byt
   switch (codec) {
     case CODEC_NONE: return chunk_bytes;
     case CODEC_LZ4_NON_STANDARD: return lz4_max_output_size(chunk_bytes);
-    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes);
+    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes, kZstdBlockMax);
     default: return 0;
   }
 }
@@ -33,7 +33,7 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
     return 1;
   }
   hipStream_t st = nullptr;
-  uint32_t nseg = codec == CODEC_ZSTD ? zstd_segments(chunk_bytes) : 1;
+  uint32_t nseg = codec == CODEC_ZSTD ? zstd_segments(chunk_bytes, kZstdBlockMax) : 1;
   uint32_t* seg = nullptr;
   void* scratch = nullptr;
   uint32_t* queue = nullptr;
@@ -51,11 +51,13 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
       goto done;
   } else if (codec == CODEC_ZSTD) {
     const uint32_t workers = (uint32_t)(n * nseg < 2048 ? n * nseg : 2048);
+    if (zstd_prepare() != hipSuccess)
+      goto done;
     if (hipMalloc(&scratch, zstd_scratch_bytes(workers)) != hipSuccess ||
         hipMalloc((void**)&queue, 256) != hipSuccess)
       goto done;
     if (launch_zstd((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
-                    seg, level, scratch, workers, queue, st) != hipSuccess)
+                    seg, level, kZstdBlockMax, scratch, workers, queue, st) != hipSuccess)
       goto done;
   } else {
     set_error("chucky_gpu_compress: unsupported codec");
@@ -75,7 +77,7 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
       uint64_t off = h[i * nseg];
       for (uint32_t k = 1; k < nseg; ++k) {
         uint8_t* base = (uint8_t*)d_dst + i * dst_stride;
-        if (hipMemcpy(base + off, base + (uint64_t)k * zstd_segment_stride(), h[i * nseg + k],
+        if (hipMemcpy(base + off, base + (uint64_t)k * zstd_segment_stride(kZstdBlockMax), h[i * nseg + k],
                       hipMemcpyDeviceToDevice) != hipSuccess)
           goto done;
         off += h[i * nseg + k];

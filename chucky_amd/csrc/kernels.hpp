@@ -56,21 +56,30 @@ hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gat
 // ---- codecs ----
 
 size_t lz4_max_output_size(size_t chunk_bytes);
-size_t zstd_max_output_size(size_t chunk_bytes);
-uint32_t zstd_segments(size_t chunk_bytes);
-uint64_t zstd_segment_stride();
+// zstd block sizes: powers of two in [kZstdBlockMin, kZstdBlockMax]; each
+// block is compressed by one wavefront, so the block size sets the number of
+// wavefronts per chunk.
+constexpr uint32_t kZstdBlockMin = 8 * 1024;
+constexpr uint32_t kZstdBlockMax = 128 * 1024;
+bool zstd_block_ok(uint32_t block);
+size_t zstd_max_output_size(size_t chunk_bytes, uint32_t block);
+uint32_t zstd_segments(size_t chunk_bytes, uint32_t block);
+uint64_t zstd_segment_stride(uint32_t block);
 
 // LZ4 raw block per chunk: one wavefront per chunk.
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                       uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
                       hipStream_t stream);
 
-// zstd frame per chunk, split into independent 128 KiB blocks (segments).
+// zstd frame per chunk, split into independent blocks (segments) of `block` bytes.
+// Uploads the predefined FSE tables to the current device (once per device;
+// call before the first launch_zstd on a device, outside graph capture).
+hipError_t zstd_prepare();
 // scratch: see zstd_scratch_bytes(workers).
 size_t zstd_scratch_bytes(uint32_t workers);
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
-                       void* scratch, uint32_t workers, uint32_t* queue_counter,
+                       uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
                        hipStream_t stream);
 
 } // namespace chucky

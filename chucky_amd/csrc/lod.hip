@@ -318,7 +318,10 @@ int upload(tile_stream_gpu* s, T** d, const std::vector<T>& h)
     return 1;
   }
   s->allocations.push_back(p);
-  if (bytes && hipMemcpy(p, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+  // copies go through the stream's own (non-blocking) queue: legacy-stream
+  // operations are refused while another thread captures a graph
+  if (bytes && (hipMemcpyAsync(p, h.data(), bytes, hipMemcpyHostToDevice, s->st_comp) != hipSuccess ||
+                hipStreamSynchronize(s->st_comp) != hipSuccess)) {
     set_error("lod: upload failed");
     return 1;
   }
@@ -348,8 +351,8 @@ int lod_init(tile_stream_gpu* s)
   s->allocations.push_back(lod);
   s->d_linear = (uint8_t*)lin;
   s->d_lod = (uint8_t*)lod;
-  if (hipMemset(lin, 0, p.levels[0].layout.epoch_elements * bpe) != hipSuccess ||
-      hipMemset(lod, 0, p.level_span_ends[p.nlod - 1] * bpe) != hipSuccess)
+  if (hipMemsetAsync(lin, 0, p.levels[0].layout.epoch_elements * bpe, s->st_comp) != hipSuccess ||
+      hipMemsetAsync(lod, 0, p.level_span_ends[p.nlod - 1] * bpe, s->st_comp) != hipSuccess)
     return 1;
   if (p.append_downsample && p.append_accum_elements()) {
     void* acc = nullptr;
@@ -357,7 +360,7 @@ int lod_init(tile_stream_gpu* s)
       return 1;
     s->allocations.push_back(acc);
     s->d_accum = (uint8_t*)acc;
-    if (hipMemset(acc, 0, p.append_accum_elements() * bpe) != hipSuccess)
+    if (hipMemsetAsync(acc, 0, p.append_accum_elements() * bpe, s->st_comp) != hipSuccess)
       return 1;
   }
   if (upload(s, &s->d_gather_lut, s->luts.gather_lut) || upload(s, &s->d_gather_fixed, s->luts.gather_fixed))

@@ -70,6 +70,7 @@ struct chucky_server
   size_t in_use = 0, peak_bytes = 0;
   uint64_t submitted = 0, finished = 0;
   int failures = 0;
+  std::string error; // first failure, reported by wait_all
   bool stop = false;
   std::thread scheduler;
 
@@ -90,6 +91,8 @@ struct chucky_server
     int status = 1;
     std::vector<double> lat;
     tile_stream_gpu* s = tile_stream_gpu_create_ex(&r.config, r.sink, &o);
+    if (!s)
+      chucky::set_error(std::string("create failed: ") + chucky_last_error());
     if (s) {
       status = 0;
       const size_t epoch_bytes =
@@ -103,14 +106,20 @@ struct chucky_server
         if (res.error == writer_error_finished)
           break;
         status = res.error;
+        if (status)
+          chucky::set_error(std::string("append failed at byte ") + std::to_string(off) + ": " +
+                            chucky_last_error());
       }
-      if (status == 0 && writer_flush(w).error)
+      if (status == 0 && writer_flush(w).error) {
         status = 1;
+        chucky::set_error(std::string("flush failed: ") + chucky_last_error());
+      }
       lat.resize(1 << 16);
       lat.resize(tile_stream_gpu_epoch_latencies(s, lat.data(), lat.size()));
       tile_stream_gpu_destroy(s);
     }
     rep.status = status;
+    const std::string why = status ? std::string(chucky_last_error()) : std::string();
     rep.end_s = since(Clock::now());
     rep.epochs = lat.size();
     rep.p50_latency_s = pct(lat, 0.50);
@@ -119,6 +128,8 @@ struct chucky_server
     std::lock_guard<std::mutex> lk(mu);
     reports.push_back(rep);
     latencies.insert(latencies.end(), lat.begin(), lat.end());
+    if (status && error.empty())
+      error = "request " + std::to_string(r.id) + ": " + (why.empty() ? "failed" : why);
     failures += status != 0;
     --running;
     in_use -= r.estimate;
@@ -197,6 +208,8 @@ int chucky_server_wait_all(struct chucky_server* srv)
 {
   std::unique_lock<std::mutex> lk(srv->mu);
   srv->cv.wait(lk, [&] { return srv->finished == srv->submitted; });
+  if (srv->failures)
+    chucky::set_error(srv->error);
   return srv->failures ? 1 : 0;
 }
 

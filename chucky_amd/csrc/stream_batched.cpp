@@ -66,11 +66,12 @@ struct BatchedState
     uint64_t epoch;
     uint32_t pool_slot, host_slot, mask;
     hipEvent_t done;
+    hipEvent_t c0, c1; // compress graph start / end (timing events)
     std::chrono::steady_clock::time_point t0;
   };
   std::deque<Ticket> inflight;
   uint64_t next_epoch = 0;
-  std::vector<hipEvent_t> event_pool;
+  std::vector<hipEvent_t> event_pool, timing_pool;
 };
 
 namespace {
@@ -129,6 +130,19 @@ hipEvent_t take_event(BatchedState* b)
   return e;
 }
 
+hipEvent_t take_timing_event(BatchedState* b)
+{
+  if (!b->timing_pool.empty()) {
+    hipEvent_t e = b->timing_pool.back();
+    b->timing_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess)
+    return nullptr;
+  return e;
+}
+
 int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
 {
   const auto codec = s->plan.config.codec;
@@ -140,8 +154,8 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
   else if (codec.id == CODEC_ZSTD)
     BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                        s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
-                       s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->d_zscratch,
-                       s->zworkers, s->d_queue, st));
+                       s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->zblock,
+                       s->d_zscratch, s->zworkers, s->d_queue, st));
   return 0;
 }
 
@@ -170,6 +184,7 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
 template<typename F>
 int capture(hipStream_t st, hipGraphExec_t* exec, F fn)
 {
+  std::lock_guard<std::recursive_mutex> lk(graph_mutex());
   hipGraph_t g = nullptr;
   BCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   const int rc = fn(st);
@@ -204,6 +219,16 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     BCHECK(q);
   }
   auto& H = b->host[t.host_slot];
+  if (t.c0 && t.c1) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.c0, t.c1) == hipSuccess)
+      metric_add(s->metrics.compress, ms, (double)s->plan.total_chunks * s->chunk_bytes, 0);
+  }
+  if (t.c0)
+    b->timing_pool.push_back(t.c0);
+  if (t.c1)
+    b->timing_pool.push_back(t.c1);
+  scatter_collect_ready(s);
   const auto t1 = std::chrono::steady_clock::now();
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     if (!(t.mask & (1u << lv)))
@@ -279,8 +304,9 @@ int batched_init(tile_stream_gpu* s)
       pm.assign(M, 0);
       const uint32_t pe[1] = { ps };
       batch_luts(p, lv, 1, pe, g.data(), pm.data());
-      BCHECK(hipMemcpy(P.d_gather[lv], g.data(), M * 4, hipMemcpyHostToDevice));
-      BCHECK(hipMemcpy(P.d_perm[lv], pm.data(), M * 4, hipMemcpyHostToDevice));
+      BCHECK(hipMemcpyAsync(P.d_gather[lv], g.data(), M * 4, hipMemcpyHostToDevice, s->st_comp));
+      BCHECK(hipMemcpyAsync(P.d_perm[lv], pm.data(), M * 4, hipMemcpyHostToDevice, s->st_comp));
+      BCHECK(hipStreamSynchronize(s->st_comp));
     }
     BCHECK(hipEventCreateWithFlags(&P.ev_compressed, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
@@ -309,9 +335,16 @@ void batched_free(tile_stream_gpu* s)
   }
   for (auto& kv : b->agg_exec)
     (void)hipGraphExecDestroy(kv.second);
-  for (auto& t : b->inflight)
+  for (auto& t : b->inflight) {
     (void)hipEventDestroy(t.done);
+    if (t.c0)
+      (void)hipEventDestroy(t.c0);
+    if (t.c1)
+      (void)hipEventDestroy(t.c1);
+  }
   for (auto e : b->event_pool)
+    (void)hipEventDestroy(e);
+  for (auto e : b->timing_pool)
     (void)hipEventDestroy(e);
   if (b->st_agg)
     (void)hipStreamDestroy(b->st_agg);
@@ -335,10 +368,21 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     for (int lv = 0; lv < s->plan.nlod; ++lv)
       s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   auto& P = b->pool[pool_slot];
+  s->slot_compressed[pool_slot] = 1;
   // compressed slots of this pool slot are free once its last gather is done
   BCHECK(hipStreamWaitEvent(s->st_comp, P.ev_gathered, 0));
-  if (P.compress_exec)
+  hipEvent_t c0 = nullptr, c1 = nullptr;
+  if (P.compress_exec) {
+    c0 = take_timing_event(b);
+    c1 = take_timing_event(b);
+    if (!c0 || !c1) {
+      set_error("batched: event creation failed");
+      return 1;
+    }
+    BCHECK(hipEventRecord(c0, s->st_comp));
     BCHECK(hipGraphLaunch(P.compress_exec, s->st_comp));
+    BCHECK(hipEventRecord(c1, s->st_comp));
+  }
   BCHECK(hipEventRecord(P.ev_compressed, s->st_comp));
   BCHECK(hipStreamWaitEvent(b->st_agg, P.ev_compressed, 0));
   const auto key = std::make_tuple(pool_slot, hs, mask);
@@ -357,7 +401,7 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     return 1;
   }
   BCHECK(hipEventRecord(done, b->st_agg));
-  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, std::chrono::steady_clock::now() });
+  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, c0, c1, std::chrono::steady_clock::now() });
   // opportunistic in-order delivery of whatever has finished
   for (;;) {
     const int r = deliver_one(s, b, false);
@@ -366,6 +410,16 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     if (r == 2 || b->inflight.empty())
       break;
   }
+  return 0;
+}
+
+// The next epoch will be written into pool slot `pool_slot` (transpose, LOD
+// scatter): queue a wait on the compute queue until the slot's previous
+// epoch has been gathered. Uncompressed streams gather straight from the pool,
+// so without it a slot could be overwritten while the aggregate queue reads it.
+int batched_acquire_slot(tile_stream_gpu* s, uint32_t pool_slot)
+{
+  BCHECK(hipStreamWaitEvent(s->st_comp, s->batched->pool[pool_slot].ev_gathered, 0));
   return 0;
 }
 
@@ -381,6 +435,7 @@ int batched_drain(tile_stream_gpu* s)
         s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   BCHECK(hipStreamSynchronize(s->st_comp));
   BCHECK(hipStreamSynchronize(b->st_agg));
+  scatter_collect_ready(s);
   return 0;
 }
 

@@ -36,7 +36,8 @@ size_t gpu_max_output(enum compression_codec c, size_t chunk_bytes)
   switch (c) {
     case CODEC_NONE: return chunk_bytes;
     case CODEC_LZ4_NON_STANDARD: return lz4_max_output_size(chunk_bytes);
-    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes);
+    // bound over every block size the stream may pick
+    case CODEC_ZSTD: return zstd_max_output_size(chunk_bytes, kZstdBlockMin);
     default: return 0;
   }
 }
@@ -143,7 +144,7 @@ int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
                         s->chunk_slot, s->d_seg_sizes, s->lz4_hash_log, s->st_comp));
   } else {
     HIPCHECK(launch_zstd(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
-                         s->chunk_slot, s->d_seg_sizes, codec.level, s->d_zscratch, s->zworkers,
+                         s->chunk_slot, s->d_seg_sizes, codec.level, s->zblock, s->d_zscratch, s->zworkers,
                          s->d_queue, s->st_comp));
   }
   HIPCHECK(hipEventRecord(e1, s->st_comp));
@@ -156,7 +157,8 @@ int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
 
 // ---- aggregate + D2H + delivery of one level ----
 
-int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t* pool_epochs)
+int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t* pool_epochs,
+                    bool empty = false)
 {
   const LevelInfo& L = s->plan.levels[lv];
   const uint64_t M = L.agg.chunks_per_epoch, C = L.agg.covering_count;
@@ -176,7 +178,13 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
   HIPCHECK(hipEventCreate(&e1));
   HIPCHECK(hipEventCreate(&e2));
   HIPCHECK(hipEventRecord(e0, s->st_comp));
-  const SegmentSource src = segment_source(s);
+  SegmentSource src = segment_source(s);
+  if (empty) {
+    // every chunk of the batch has size 0 (an index of empty entries)
+    src = SegmentSource{};
+    src.base = s->d_pool;
+    src.nseg = 1;
+  }
   const uint64_t group = (uint64_t)active * L.agg.cps_inner;
   HIPCHECK(launch_aggregate_plan(src, s->d_gather, s->d_perm, n_items, n_slots, group,
                                  s->plan.config.shard_alignment, s->d_chunk_sizes, s->d_offsets,
@@ -238,6 +246,23 @@ int scatter_end(tile_stream_gpu* s, hipEvent_t a, double bytes)
   return 0;
 }
 
+// Records the scatter launches that have completed, keeps the others.
+void collect_scatter_ready(tile_stream_gpu* s)
+{
+  size_t keep = 0;
+  for (size_t i = 0; i < s->pending_scatter.size(); ++i) {
+    auto& t = s->pending_scatter[i];
+    if (hipEventQuery(t.b) == hipSuccess) {
+      add_metric(s->metrics.scatter, event_ms(t.a, t.b), t.bytes, t.bytes);
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    } else {
+      s->pending_scatter[keep++] = t;
+    }
+  }
+  s->pending_scatter.resize(keep);
+}
+
 void collect_scatter(tile_stream_gpu* s)
 {
   for (auto& t : s->pending_scatter) {
@@ -255,6 +280,8 @@ int flush_batch(tile_stream_gpu* s, uint32_t n_epochs, const uint32_t* masks)
 {
   if (compress_batch(s, n_epochs))
     return 1;
+  for (uint32_t e = 0; e < n_epochs; ++e)
+    s->slot_compressed[e] = 1;
   collect_scatter(s);
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     uint32_t pool_epochs[MAX_BATCH_EPOCHS];
@@ -273,6 +300,35 @@ int flush_batch(tile_stream_gpu* s, uint32_t n_epochs, const uint32_t* masks)
     const auto& t0 = s->epoch_t0[s->epochs_written % s->epoch_t0.size()];
     s->latencies.push_back(std::chrono::duration<double>(now - t0).count());
     ++s->epochs_written;
+  }
+  return 0;
+}
+
+// The one-epoch batch that delivers the drained dim-0 accumulators (pool
+// slot 0). src/cpu/pipeline.c:159-180 reuses a level's init-time batch LUT
+// (pool epoch (a+1)*period-1, src/cpu/pipeline.c:14-31) whenever the active
+// count equals the level's batch_active_count = K / period; for this batch
+// that is the case exactly when period == K, and the reference then delivers
+// pool slot K-1 as last compressed (the level's previous emission), or
+// zero-size chunks when no batch has compressed that slot yet.
+int flush_drain(tile_stream_gpu* s, uint32_t mask)
+{
+  if (compress_batch(s, 1))
+    return 1;
+  s->slot_compressed[0] = 1;
+  collect_scatter(s);
+  const uint32_t K = s->plan.epochs_per_batch;
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    if (!(mask & (1u << lv)))
+      continue;
+    uint32_t pe[1] = { 0 };
+    bool empty = false;
+    if ((1u << lv) == K) {
+      pe[0] = K - 1;
+      empty = !s->slot_compressed[K - 1];
+    }
+    if (aggregate_level(s, lv, 1, pe, empty))
+      return 1;
   }
   return 0;
 }
@@ -368,6 +424,8 @@ int end_epoch(tile_stream_gpu* s)
     if (batched_submit(s, s->batch_acc, mask))
       return 1;
     s->batch_acc = (s->batch_acc + 1) % s->plan.epochs_per_batch;
+    if (batched_acquire_slot(s, s->batch_acc))
+      return 1;
   } else if (s->batch_acc >= MAX_BATCH_EPOCHS) {
     set_error("batch overflow");
     return 1;
@@ -427,11 +485,8 @@ int flush_all(tile_stream_gpu* s)
     uint32_t drain = 0;
     if (lod_drain(s, &drain))
       return 1;
-    if (drain) {
-      uint32_t m[1] = { drain };
-      if (flush_batch(s, 1, m))
-        return 1;
-    }
+    if (drain && flush_drain(s, drain))
+      return 1;
   }
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     if (s->sink->wait_fence)
@@ -563,12 +618,21 @@ void free_stream(tile_stream_gpu* s)
   delete s;
 }
 
-uint32_t default_workers()
+uint32_t default_workers() { return compressor_slots(); }
+
+// Chunks handed to one compressor launch: one epoch in the batched pipeline,
+// a batch of K epochs (at most the stream's length when dim 0 is bounded)
+// in the plain one.
+uint64_t chunks_per_launch(const tile_stream_gpu* s)
 {
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  return (uint32_t)cus * 8;
+  const Plan& p = s->plan;
+  if (s->pipeline == CHUCKY_PIPELINE_BATCHED && p.config.shard_alignment == 0)
+    return p.total_chunks;
+  uint64_t epochs = p.epochs_per_batch;
+  const uint64_t E = p.levels[0].layout.epoch_elements;
+  if (p.max_cursor_elements && E)
+    epochs = std::min<uint64_t>(epochs, (p.max_cursor_elements + E - 1) / E);
+  return std::max<uint64_t>(epochs, 1) * p.total_chunks;
 }
 
 int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, shard_sink* sink)
@@ -598,6 +662,7 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   HIPCHECK(hipStreamCreateWithFlags(&s->st_d2h, hipStreamNonBlocking));
 
   const uint32_t K = p.epochs_per_batch;
+  s->slot_compressed.assign(K, 0);
   const tile_stream_layout& L0 = p.levels[0].layout;
   s->chunk_stride_bytes = L0.chunk_stride * p.bpe;
   s->chunk_bytes = L0.chunk_elements * p.bpe;
@@ -632,9 +697,16 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     s->seg_stride = s->chunk_slot;
     s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : 12;
   } else if (config->codec.id == CODEC_ZSTD) {
-    s->nseg = zstd_segments(s->chunk_bytes);
-    s->seg_stride = zstd_segment_stride();
-    s->chunk_slot = zstd_max_output_size(s->chunk_bytes);
+    HIPCHECK(zstd_prepare());
+    if (s->opts.zstd_block_bytes && !zstd_block_ok(s->opts.zstd_block_bytes)) {
+      set_error("create: zstd_block_bytes must be a power of two in [8 KiB, 128 KiB]");
+      return 1;
+    }
+    s->zblock = s->opts.zstd_block_bytes ? s->opts.zstd_block_bytes
+                                         : tune_zstd_block(s->chunk_bytes, chunks_per_launch(s));
+    s->nseg = zstd_segments(s->chunk_bytes, s->zblock);
+    s->seg_stride = zstd_segment_stride(s->zblock);
+    s->chunk_slot = zstd_max_output_size(s->chunk_bytes, s->zblock);
     s->zworkers = s->opts.compress_workers ? s->opts.compress_workers : default_workers();
     const uint64_t blocks = nchunks * s->nseg;
     if (s->zworkers > blocks)
@@ -686,8 +758,11 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   if (dalloc(s, (void**)&s->layout_gpu.d_lifted_shape, MAX_RANK * 8) ||
       dalloc(s, (void**)&s->layout_gpu.d_lifted_strides, MAX_RANK * 8))
     return 1;
-  HIPCHECK(hipMemcpy(s->layout_gpu.d_lifted_shape, L0.lifted_shape, MAX_RANK * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(s->layout_gpu.d_lifted_strides, L0.lifted_strides, MAX_RANK * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpyAsync(s->layout_gpu.d_lifted_shape, L0.lifted_shape, MAX_RANK * 8,
+                          hipMemcpyHostToDevice, s->st_comp));
+  HIPCHECK(hipMemcpyAsync(s->layout_gpu.d_lifted_strides, L0.lifted_strides, MAX_RANK * 8,
+                          hipMemcpyHostToDevice, s->st_comp));
+  HIPCHECK(hipStreamSynchronize(s->st_comp));
 
   s->metrics.memcpy = mk_metric("memcpy");
   s->metrics.h2d = mk_metric("h2d");
@@ -712,6 +787,19 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
 
 } // namespace
 
+namespace chucky {
+std::recursive_mutex& graph_mutex()
+{
+  static std::recursive_mutex m;
+  return m;
+}
+void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes)
+{
+  add_metric(m, ms, in_bytes, out_bytes);
+}
+void scatter_collect_ready(tile_stream_gpu* s) { collect_scatter_ready(s); }
+} // namespace chucky
+
 extern "C" {
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o)
@@ -725,6 +813,7 @@ struct tile_stream_gpu* tile_stream_gpu_create_ex(const struct tile_stream_confi
                                                   struct shard_sink* sink,
                                                   const struct tile_stream_gpu_options* opts)
 {
+  std::lock_guard<std::recursive_mutex> lk(graph_mutex());
   tile_stream_gpu* s = new (std::nothrow) tile_stream_gpu();
   if (!s)
     return nullptr;
@@ -746,7 +835,11 @@ struct tile_stream_gpu* tile_stream_gpu_create(const struct tile_stream_configur
   return tile_stream_gpu_create_ex(config, sink, nullptr);
 }
 
-void tile_stream_gpu_destroy(struct tile_stream_gpu* stream) { free_stream(stream); }
+void tile_stream_gpu_destroy(struct tile_stream_gpu* stream)
+{
+  std::lock_guard<std::recursive_mutex> lk(graph_mutex());
+  free_stream(stream);
+}
 
 struct stream_metrics tile_stream_gpu_get_metrics(const struct tile_stream_gpu* s) { return s->metrics; }
 
@@ -791,6 +884,7 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   s->cursor = 0;
   s->batch_acc = 0;
   s->flushed = 0;
+  std::fill(s->slot_compressed.begin(), s->slot_compressed.end(), (uint8_t)0);
   s->stage_fill = 0;
   std::fill(std::begin(s->append_counts), std::end(s->append_counts), 0u);
   for (auto& ss : s->shards)
@@ -861,10 +955,10 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
   }
   info->lod_bytes = lod;
   if (config->codec.id == CODEC_ZSTD) {
-    const uint64_t blocks = nchunks * zstd_segments(cb);
+    const uint64_t blocks = nchunks * zstd_segments(cb, kZstdBlockMin);
     info->codec_bytes = zstd_scratch_bytes((uint32_t)std::min<uint64_t>(blocks, 2048));
   }
-  info->codec_bytes += nchunks * (config->codec.id == CODEC_ZSTD ? zstd_segments(cb) : 1) * 4;
+  info->codec_bytes += nchunks * (config->codec.id == CODEC_ZSTD ? zstd_segments(cb, kZstdBlockMin) : 1) * 4;
   uint64_t shard = 0;
   for (int lv = 0; lv < p.nlod; ++lv)
     shard += p.levels[lv].shard_inner_count * (sizeof(OpenShard) + p.levels[lv].cps_total * 16);

@@ -13,6 +13,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <mutex>
 #include <vector>
 
 namespace chucky {
@@ -53,6 +54,7 @@ struct tile_stream_gpu
   uint64_t seg_stride = 0, chunk_slot = 0;
   void* d_zscratch = nullptr;
   uint32_t zworkers = 0;
+  uint32_t zblock = chucky::kZstdBlockMax;
   uint32_t* d_queue = nullptr;
   int lz4_hash_log = 12;
 
@@ -79,6 +81,7 @@ struct tile_stream_gpu
   std::vector<uint32_t*> d_chunk_lut;
   std::vector<uint64_t*> d_chunk_fixed;
   uint32_t append_counts[LOD_MAX_LEVELS] = {};
+  std::vector<uint8_t> slot_compressed; // per pool slot: compressed by a batch since the start
 
   // stream state
   uint64_t cursor = 0;
@@ -117,9 +120,18 @@ void set_error(const std::string& msg);
 int lod_run_epoch(tile_stream_gpu* s, uint32_t epoch_in_batch, uint32_t* out_mask);
 int lod_drain(tile_stream_gpu* s, uint32_t* out_mask);
 int lod_init(tile_stream_gpu* s);
+void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes);
+// Held while a graph is captured and while streams are created or destroyed,
+// so that one thread's capture never sees another thread's allocations,
+// frees or initial copies.
+std::recursive_mutex& graph_mutex();
+void scatter_collect_ready(tile_stream_gpu* s);
+uint32_t compressor_slots();
+uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch);
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
 int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);
+int batched_acquire_slot(tile_stream_gpu* s, uint32_t pool_slot);
 int batched_drain(tile_stream_gpu* s);
 void batched_reset(tile_stream_gpu* s);
 } // namespace chucky

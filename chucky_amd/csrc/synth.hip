@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,35 @@ __global__ __launch_bounds__(256) void sensor_k(uint16_t* __restrict__ out, uint
     sensor_pixel(out, i, per_frame, W, seed, cdf, cdf_len, blobs, nblob);
 }
 
+// Seeded fill: integers uniform over [0, range) (range 0 = all bit patterns of
+// the type), floats N(0, 1) * 100 by the Box-Muller transform of two uniforms.
+template<typename T>
+__global__ __launch_bounds__(256) void fill_k(T* __restrict__ out, uint64_t n, uint64_t seed,
+                                              uint64_t range)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    const uint64_t r = mix64(seed + (i + 1) * 0x9E3779B97F4A7C15ull);
+    if constexpr (std::is_floating_point<T>::value || std::is_same<T, _Float16>::value) {
+      const uint64_t r2 = mix64(r ^ 0xD1B54A32D192ED03ull);
+      const float u1 = ((float)(r >> 40) + 0.5f) * (1.f / 16777216.f);
+      const float u2 = (float)(r2 >> 40) * (1.f / 16777216.f);
+      const float z = sqrtf(-2.f * logf(u1)) * cospif(2.f * u2);
+      out[i] = (T)(z * 100.f);
+    } else {
+      out[i] = (T)(range ? r % range : r);
+    }
+  }
+}
+
+template<typename T>
+int fill_t(void* d, uint64_t n, uint64_t seed, uint64_t range)
+{
+  hipLaunchKernelGGL(fill_k<T>, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 1u << 20)),
+                     dim3(256), 0, nullptr, (T*)d, n, seed, range);
+  return hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess ? 0 : 1;
+}
+
 } // namespace
 
 extern "C" {
@@ -87,14 +117,26 @@ void* chucky_device_alloc(size_t bytes)
 
 void chucky_device_free(void* p) { (void)hipFree(p); }
 
+// Copies run on a non-blocking queue of the calling thread: legacy-stream
+// copies are refused while another thread captures a graph.
+static int copy_sync(void* dst, const void* src, size_t n, hipMemcpyKind kind)
+{
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return 1;
+  const bool ok = hipMemcpyAsync(dst, src, n, kind, st) == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+  (void)hipStreamDestroy(st);
+  return ok ? 0 : 1;
+}
+
 int chucky_memcpy_to_device(void* d, const void* h, size_t n)
 {
-  return hipMemcpy(d, h, n, hipMemcpyHostToDevice) == hipSuccess ? 0 : 1;
+  return copy_sync(d, h, n, hipMemcpyHostToDevice);
 }
 
 int chucky_memcpy_to_host(void* h, const void* d, size_t n)
 {
-  return hipMemcpy(h, d, n, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+  return copy_sync(h, d, n, hipMemcpyDeviceToHost);
 }
 
 int chucky_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
@@ -131,6 +173,25 @@ done:
   (void)hipFree(d_cdf);
   (void)hipFree(d_blobs);
   return rc;
+}
+
+int chucky_synth_fill(void* d_dst, uint64_t n, enum dtype dt, uint64_t seed, uint64_t int_range)
+{
+  switch (dt) {
+    case dtype_u8: return fill_t<uint8_t>(d_dst, n, seed, int_range);
+    case dtype_u16: return fill_t<uint16_t>(d_dst, n, seed, int_range);
+    case dtype_u32: return fill_t<uint32_t>(d_dst, n, seed, int_range);
+    case dtype_u64: return fill_t<uint64_t>(d_dst, n, seed, int_range);
+    case dtype_i8: return fill_t<int8_t>(d_dst, n, seed, int_range);
+    case dtype_i16: return fill_t<int16_t>(d_dst, n, seed, int_range);
+    case dtype_i32: return fill_t<int32_t>(d_dst, n, seed, int_range);
+    case dtype_i64: return fill_t<int64_t>(d_dst, n, seed, int_range);
+    case dtype_f16: return fill_t<_Float16>(d_dst, n, seed, int_range);
+    case dtype_f32: return fill_t<float>(d_dst, n, seed, int_range);
+    case dtype_f64: return fill_t<double>(d_dst, n, seed, int_range);
+  }
+  chucky::set_error("chucky_synth_fill: unknown dtype");
+  return 1;
 }
 
 } // extern "C"

@@ -24,6 +24,21 @@ __device__ __forceinline__ uint32_t ld32u(const uint8_t* p)
   return (uint32_t)((((uint64_t)w[1] << 32) | lo) >> sh);
 }
 
+// 8 bytes at any address, from three aligned dword loads (same slack rule).
+__device__ __forceinline__ uint64_t ld64u(const uint8_t* p)
+{
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+  const uint32_t sh = (uint32_t)(a & 3) * 8;
+  const uint32_t w0 = w[0], w1 = w[1];
+  if (sh == 0)
+    return ((uint64_t)w1 << 32) | w0;
+  const uint32_t w2 = w[2];
+  const uint32_t lo = (uint32_t)((((uint64_t)w1 << 32) | w0) >> sh);
+  const uint32_t hi = (uint32_t)((((uint64_t)w2 << 32) | w1) >> sh);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Wave-cooperative copy of n bytes (any alignment): byte head to a dword
 // boundary of dst, dword body (funnel-shifted source), byte tail.
 __device__ __forceinline__ void wave_copy(uint8_t* dst, const uint8_t* src, uint32_t n)

@@ -8,12 +8,16 @@
 // each segment at a fixed stride; the aggregate gather concatenates them.
 //
 // Per block (RFC 8878 3.1.1):
-//  * greedy wave-parallel LZ77 parse: 64 candidate positions per step, a
-//    6-byte key hashed into an LDS table, matches of >= 6 bytes inside the
+//  * greedy wave-parallel LZ77 parse: 64 candidate positions per step, an
+//    8-byte key hashed into an LDS table, matches of >= 8 bytes inside the
 //    block; after long literal runs the 64 lanes spread out (zstd-style
 //    search acceleration: stride 1 + run/256), so incompressible stretches
-//    cost little; literal runs are compacted to scratch while a single
-//    256-bin LDS histogram counts them;
+//    cost little. The 8-byte minimum is deliberate: on noisy integer data
+//    (sensor frames, small-range integers) 6- and 7-byte matches cost more
+//    in the sequence section than their Huffman-coded literals, and every
+//    sequence is serial work; literal runs are compacted to scratch while a
+//    single 256-bin LDS histogram counts them (a block without matches is
+//    its own literal buffer);
 //  * RLE block when the parse finds one offset-1 match covering the block;
 //  * literals: Huffman, 4 streams, table built per block (FSE-compressed or
 //    direct weights); each stream is encoded by the whole wave (per-lane bit
@@ -28,42 +32,52 @@
 #include "wave.hpp"
 #include "zstd_core.hpp"
 
+#include <map>
+#include <mutex>
+
 namespace chucky {
 namespace {
 
 using zcore::Seq;
 
-constexpr uint32_t kBlock = zcore::kBlockMax;
-constexpr uint64_t kSegStride = kBlock + 64; // >= frame header + block header + raw block
+constexpr uint32_t kBlock = zcore::kBlockMax; // largest block (scratch is sized for it)
+constexpr uint64_t kSegSlack = 64; // segment = block + slack >= frame header + block header + raw block
 constexpr int kHashLog = 12;
-constexpr uint32_t kMinMatch = 6;
-constexpr uint32_t kMaxSeq = 16384;
+constexpr uint32_t kMinMatch = 8;
+constexpr uint32_t kMaxSeq = 8192;
 constexpr uint64_t kLitsBytes = kBlock + 256;
 constexpr uint64_t kSeqBytes = (uint64_t)kMaxSeq * sizeof(Seq);
 constexpr uint64_t kSeqOutBytes = (uint64_t)kMaxSeq * 12 + 256;
 constexpr uint64_t kWorkerScratch = (kLitsBytes + kSeqBytes + kSeqOutBytes + 255) & ~255ull;
-constexpr int kStageWords = 104;
-constexpr int kSymPerLane = 4;
+constexpr int kSymPerLane = 8; // Huffman symbols per lane per round
+// one round: <= 64 lanes * 8 symbols * 11 bits, plus < 8 carried bits
+constexpr int kStageWords = (kWave * kSymPerLane * zcore::kHufMaxBits + 8 + 31) / 32 + 4;
 
+// ~10.2 KiB per wavefront, so that 15 wavefronts fit a CU's LDS. The union
+// is the parse's hash table, then the Huffman builder's scratch, then the
+// literal encoder's packed codes and bit staging.
 struct Lds
 {
   union
   {
-    uint32_t table[1 << kHashLog];
+    uint16_t table[1 << kHashLog]; // low 16 bits of the last position per hash
     zcore::HufWork huf;
+    struct
+    {
+      uint32_t cw[256]; // code value | length << 16
+      uint32_t stage[kStageWords];
+    } enc;
   } u;
   uint32_t hist[256];
   zcore::HufCode code;
-  zcore::Predefined pre;
   uint8_t tree[192];
-  uint32_t stage[kStageWords + 4];
   // decisions published by lane 0
   uint32_t mode, tree_size, lit_size, lit_hsize, seq_size;
 };
 
-__device__ __forceinline__ uint32_t key_hash(uint64_t key48)
+__device__ __forceinline__ uint32_t key_hash(uint64_t key)
 {
-  return (uint32_t)((key48 * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
+  return (uint32_t)((key * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
 }
 
 // Ascending bitonic sort of 256 keys in LDS by one wavefront (two
@@ -87,63 +101,115 @@ __device__ void wave_sort256(uint32_t* keys)
     }
 }
 
+// Adds every byte of src[0, n) to the LDS histogram. Each lane keeps one
+// (symbol, run) pair per byte position within a dword and only issues an LDS
+// atomic when the symbol changes: the high bytes of small 16/32-bit integers
+// (most often 0) would otherwise make every atomic of the wave hit one bin.
+__device__ __forceinline__ void count_bytes(const uint8_t* src, uint32_t n, uint32_t* hist)
+{
+  const int lane = lane_id();
+  uint32_t sym[4] = { 0, 0, 0, 0 }, run[4] = { 0, 0, 0, 0 };
+  const uint32_t body = n & ~63u; // whole 16-byte pieces per lane
+  for (uint32_t i = 16u * lane; i < body; i += 16u * kWave) {
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      w[j] = ld32u(src + i + 4 * j);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t b = (w[j] >> (8 * k)) & 0xFF;
+        if (b == sym[k]) {
+          ++run[k];
+        } else {
+          if (run[k])
+            atomicAdd(&hist[sym[k]], run[k]);
+          sym[k] = b;
+          run[k] = 1;
+        }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (run[k])
+      atomicAdd(&hist[sym[k]], run[k]);
+  for (uint32_t i = body + lane; i < n; i += kWave)
+    atomicAdd(&hist[src[i]], 1u);
+}
+
 // Compacts in[0, n) to dst and adds every byte to the LDS histogram.
 __device__ __forceinline__ void copy_count(uint8_t* dst, const uint8_t* src, uint32_t n,
                                            uint32_t* hist)
 {
   wave_copy(dst, src, n);
-  const int lane = lane_id();
-  for (uint32_t i = 4u * lane; i < n; i += 4u * kWave) {
-    const uint32_t w = ld32u(src + i);
-    const uint32_t m = n - i < 4 ? n - i : 4;
-    for (uint32_t k = 0; k < m; ++k)
-      atomicAdd(&hist[(w >> (8 * k)) & 0xFF], 1u);
+  count_bytes(src, n, hist);
+}
+
+// The up-to-8 symbols lane `lane` encodes in the round whose highest symbol
+// is `top`: symbols top-8*lane-7 .. top-8*lane, as bytes of a 64-bit word
+// (byte j = symbol lo+j; bytes below `a` are zero and skipped by the caller).
+__device__ __forceinline__ uint64_t round_syms(const uint8_t* lits, long a, long top, int lane,
+                                               int* skip)
+{
+  const long lo = top - (long)lane * kSymPerLane - (kSymPerLane - 1);
+  if (lo + kSymPerLane - 1 < a) {
+    *skip = kSymPerLane;
+    return 0;
   }
+  const int sk = lo >= a ? 0 : (int)(a - lo);
+  *skip = sk;
+  return ld64u(lits + lo + sk) << (8 * sk);
 }
 
 // Encodes lits[a, b) as one backward Huffman stream at out; returns its size.
+// Each round the wave takes 512 symbols, highest first: every lane packs its
+// 8 codes into a 128-bit word, a wave prefix sum of the bit counts places the
+// words, LDS atomic-ORs assemble them in `stage`, and complete bytes are
+// copied out. The next round's symbols are loaded before this round's codes
+// are looked up, so the global load latency overlaps the LDS work.
 __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t a, uint32_t b,
-                                    const zcore::HufCode& h, uint32_t* stage)
+                                    const uint32_t* cw, uint32_t* stage)
 {
   const int lane = lane_id();
-  for (int i = lane; i < kStageWords + 4; i += kWave)
+  for (int i = lane; i < kStageWords; i += kWave)
     stage[i] = 0;
   __syncthreads();
   uint32_t flushed = 0; // bytes already copied out
   uint32_t rel = 0;     // bits held in stage (stage bit 0 = stream bit 8*flushed)
   long top = (long)b - 1;
+  int skip = 0;
+  uint64_t q = round_syms(lits, a, top, lane, &skip);
   for (;;) {
-    // lane L takes symbols top - 4L - 3 .. top - 4L (one dword), highest first
-    const long lo = top - (long)lane * kSymPerLane - (kSymPerLane - 1);
-    uint64_t bits = 0;
+    const long next = top - (long)kWave * kSymPerLane;
+    int nskip = kSymPerLane;
+    const uint64_t nq = next >= (long)a ? round_syms(lits, a, next, lane, &nskip) : 0;
+    unsigned __int128 bits = 0;
     uint32_t nb = 0;
-    if (lo + kSymPerLane - 1 >= (long)a) {
-      const uint32_t w = ld32u(lits + (lo >= (long)a ? lo : (long)a));
-      const int skip = lo >= (long)a ? 0 : (int)((long)a - lo);
-      for (int j = kSymPerLane - 1; j >= skip; --j) {
-        const uint8_t s = (uint8_t)(w >> (8 * (j - skip)));
-        bits |= (uint64_t)h.val[s] << nb;
-        nb += h.len[s];
+#pragma unroll
+    for (int j = kSymPerLane - 1; j >= 0; --j) {
+      if (j >= skip) {
+        const uint32_t c = cw[(uint32_t)(q >> (8 * j)) & 0xFF];
+        bits |= (unsigned __int128)(c & 0xFFFF) << nb;
+        nb += c >> 16;
       }
     }
     const uint32_t incl = wave_incl_scan(nb);
     const uint32_t total = wave_bcast(incl, kWave - 1);
     if (nb) {
       const uint32_t p = rel + incl - nb;
-      const uint32_t w = p >> 5, sh = p & 31;
-      const uint32_t w0 = (uint32_t)(bits << sh);
-      const uint32_t w1 = sh ? (uint32_t)(bits >> (32 - sh)) : (uint32_t)(bits >> 32);
-      const uint32_t w2 = sh ? (uint32_t)(bits >> (64 - sh)) : 0u;
-      atomicOr(&stage[w], w0);
-      if (w1)
-        atomicOr(&stage[w + 1], w1);
-      if (w2)
-        atomicOr(&stage[w + 2], w2);
+      const unsigned __int128 sh = bits << (p & 31);
+      uint32_t* w = stage + (p >> 5);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (uint32_t)(sh >> (32 * k));
+        if (v)
+          atomicOr(&w[k], v);
+      }
     }
     __syncthreads();
     rel += total;
-    top -= (long)kWave * kSymPerLane;
-    if (top < (long)a) {
+    if (next < (long)a) {
       if (lane == 0)
         stage[rel >> 5] |= 1u << (rel & 31); // end mark
       __syncthreads();
@@ -154,20 +220,24 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
     }
     const uint32_t full = rel >> 3;
     wave_copy(out + flushed, (const uint8_t*)stage, full);
-    __syncthreads();
     const uint32_t carry = ((const uint8_t*)stage)[full];
     __syncthreads();
-    for (int i = lane; i < kStageWords + 4; i += kWave)
-      stage[i] = (i == 0) ? carry : 0u;
+    const uint32_t used_words = (rel + 31) >> 5;
+    for (uint32_t i = lane; i < used_words; i += kWave)
+      stage[i] = (i == 0) ? (carry & ((1u << (rel & 7)) - 1u)) : 0u;
     __syncthreads();
     flushed += full;
     rel &= 7;
+    top = next;
+    q = nq;
+    skip = nskip;
   }
 }
 
 // Compresses one block; returns bytes written at out (header included).
 __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
-                                   uint8_t* lits, Seq* seqs, uint8_t* seqout)
+                                   uint8_t* lits, Seq* seqs, uint8_t* seqout,
+                                   const zcore::Predefined& pre)
 {
   const int lane = lane_id();
 
@@ -179,25 +249,27 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
   __syncthreads();
   uint32_t nseq = 0, nlits = 0, anchor = 0;
   if (len >= 16) {
-    const uint32_t last_start = len - kMinMatch; // a key needs 6 readable bytes
+    const uint32_t last_start = len - kMinMatch; // a key needs 8 readable bytes
     uint32_t pos = 0;
     while (pos <= last_start && nseq < kMaxSeq) {
       const uint32_t stride = 1 + ((pos - anchor) >> 8);
       const uint32_t p = pos + (uint32_t)lane * stride;
       const bool active = p <= last_start;
-      uint32_t v0 = 0, v1 = 0, h = 0, cand = 0;
+      uint64_t key = 0;
+      uint32_t h = 0, dist = 0;
       if (active) {
-        v0 = ld32u(in + p);
-        v1 = ld32u(in + p + 4) & 0xFFFFu;
-        h = key_hash(((uint64_t)v1 << 32) | v0);
-        cand = L.u.table[h];
+        key = ld64u(in + p);
+        h = key_hash(key);
+        // entries hold positions mod 2^16: the candidate is the last position
+        // with this hash if it lies within 64 KiB before p. Stale, empty or
+        // foreign entries only ever name in-block positions before p, and the
+        // 8-byte compare rejects every candidate that is not a match.
+        dist = (p - (uint32_t)L.u.table[h]) & 0xFFFFu;
       }
-      // entries at or after p can be left by lanes past the previous match
-      const bool hit = active && cand != 0 && cand - 1 < p && ld32u(in + cand - 1) == v0 &&
-                       (ld32u(in + cand + 3) & 0xFFFFu) == v1;
+      const bool hit = active && dist != 0 && dist <= p && ld64u(in + p - dist) == key;
       __builtin_amdgcn_wave_barrier();
       if (active)
-        L.u.table[h] = p + 1;
+        L.u.table[h] = (uint16_t)p;
       __builtin_amdgcn_wave_barrier();
       const uint64_t mask = wave_ballot(hit);
       if (mask == 0) {
@@ -206,7 +278,7 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       }
       const int first = __ffsll((unsigned long long)mask) - 1;
       uint32_t mp = pos + (uint32_t)first * stride;
-      uint32_t mc = wave_bcast(cand, first) - 1;
+      uint32_t mc = mp - wave_bcast(dist, first);
       uint32_t ml = wave_match_extend(in, mp, mc, kMinMatch, len);
       while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
         --mp;
@@ -223,7 +295,7 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       anchor = pos;
       if (lane == 0 && pos >= 2 && pos - 2 <= last_start) {
         const uint32_t q = pos - 2;
-        L.u.table[key_hash(((uint64_t)(ld32u(in + q + 4) & 0xFFFFu) << 32) | ld32u(in + q))] = q + 1;
+        L.u.table[key_hash(ld64u(in + q))] = (uint16_t)q;
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -241,7 +313,12 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       return 4;
     }
   }
-  copy_count(lits + nlits, in + anchor, len - anchor, L.hist);
+  if (nseq == 0) {
+    count_bytes(in, len, L.hist);
+    lits = const_cast<uint8_t*>(in);
+  } else {
+    copy_count(lits + nlits, in + anchor, len - anchor, L.hist);
+  }
   nlits += len - anchor;
   __syncthreads();
 
@@ -293,7 +370,7 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
   }
   // ---- sequences section into scratch (lane 0) ----
   if (lane == 0)
-    L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, L.pre);
+    L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, pre);
   __syncthreads();
   const uint32_t mode = L.mode, seq_size = L.seq_size;
   if (L.lit_size + seq_size >= len) {
@@ -320,12 +397,15 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
     if (lane == 0)
       for (uint32_t i = 0; i < ts; ++i)
         o[hs + i] = L.tree[i];
+    for (int sy = lane; sy < 256; sy += kWave)
+      L.u.enc.cw[sy] = (uint32_t)L.code.val[sy] | ((uint32_t)L.code.len[sy] << 16);
+    __syncthreads();
     const uint32_t seg = (nlits + 3) / 4;
     uint32_t sp = hs + ts + 6, sizes[4];
     for (int k = 0; k < 4; ++k) {
       const uint32_t a = (uint32_t)k * seg;
       const uint32_t b = k == 3 ? nlits : a + seg;
-      sizes[k] = huf_stream_wave(o + sp, lits, a, b, L.code, L.stage);
+      sizes[k] = huf_stream_wave(o + sp, lits, a, b, L.u.enc.cw, L.u.enc.stage);
       sp += sizes[k];
     }
     const uint32_t comp = sp - hs;
@@ -346,18 +426,15 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
   return 3 + body;
 }
 
-__global__ __launch_bounds__(64) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
-                                             uint32_t chunk_bytes, uint32_t nseg,
+__global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
+                                             uint32_t chunk_bytes, uint32_t block, uint32_t nseg,
                                              uint64_t total_blocks, uint8_t* __restrict__ dst,
                                              uint64_t chunk_dst_stride,
                                              uint32_t* __restrict__ seg_sizes, uint8_t* scratch,
-                                             uint32_t* queue)
+                                             uint32_t* queue, const zcore::Predefined* __restrict__ pre)
 {
   __shared__ Lds L;
   const int lane = lane_id();
-  if (lane == 0)
-    zcore::build_predefined(L.pre, L.u.huf.spread, L.u.huf.cumul);
-  __syncthreads();
   uint8_t* ws = scratch + (uint64_t)blockIdx.x * kWorkerScratch;
   uint8_t* lits = ws;
   Seq* seqs = (Seq*)(ws + kLitsBytes);
@@ -374,16 +451,16 @@ __global__ __launch_bounds__(64) void zstd_k(const uint8_t* __restrict__ src, ui
       break;
     const uint64_t chunk = item / nseg;
     const uint32_t b = (uint32_t)(item % nseg);
-    const uint8_t* in = src + chunk * src_stride + (uint64_t)b * kBlock;
-    const uint32_t len = (chunk_bytes - b * kBlock) < kBlock ? chunk_bytes - b * kBlock : kBlock;
-    uint8_t* out = dst + chunk * chunk_dst_stride + (uint64_t)b * kSegStride;
+    const uint8_t* in = src + chunk * src_stride + (uint64_t)b * block;
+    const uint32_t len = (chunk_bytes - b * block) < block ? chunk_bytes - b * block : block;
+    uint8_t* out = dst + chunk * chunk_dst_stride + (uint64_t)b * (block + kSegSlack);
     uint32_t hdr = 0;
     if (b == 0) {
       if (lane == 0)
         zcore::write_frame_header(out, chunk_bytes);
       hdr = (uint32_t)zcore::kFrameHeaderBytes;
     }
-    const uint32_t sz = compress_block(L, in, len, out + hdr, b + 1 == nseg, lits, seqs, seqout);
+    const uint32_t sz = compress_block(L, in, len, out + hdr, b + 1 == nseg, lits, seqs, seqout, *pre);
     if (lane == 0)
       seg_sizes[chunk * nseg + b] = hdr + sz;
     __syncthreads();
@@ -394,36 +471,93 @@ __global__ __launch_bounds__(64) void zstd_k(const uint8_t* __restrict__ src, ui
 
 __global__ void zero_u32_k(uint32_t* p) { *p = 0; }
 
+std::mutex g_pre_mu;
+std::map<int, zcore::Predefined*> g_pre; // per device, kept until the process ends
+
+const zcore::Predefined* predefined_on_current_device()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return nullptr;
+  std::lock_guard<std::mutex> lk(g_pre_mu);
+  auto it = g_pre.find(dev);
+  return it == g_pre.end() ? nullptr : it->second;
+}
+
 } // namespace
 
-uint32_t zstd_segments(size_t chunk_bytes) { return (uint32_t)((chunk_bytes + kBlock - 1) / kBlock); }
-uint64_t zstd_segment_stride() { return kSegStride; }
-size_t zstd_max_output_size(size_t chunk_bytes) { return (size_t)zstd_segments(chunk_bytes) * kSegStride; }
+hipError_t zstd_prepare()
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return e;
+  std::lock_guard<std::mutex> lk(g_pre_mu);
+  if (g_pre.count(dev))
+    return hipSuccess;
+  zcore::Predefined h;
+  uint8_t spread[512];
+  uint32_t cumul[66];
+  zcore::build_predefined(h, spread, cumul);
+  zcore::Predefined* d = nullptr;
+  hipStream_t st = nullptr;
+  if ((e = hipMalloc((void**)&d, sizeof h)) != hipSuccess)
+    return e;
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) == hipSuccess) {
+    e = hipMemcpyAsync(d, &h, sizeof h, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return e;
+  }
+  g_pre[dev] = d;
+  return hipSuccess;
+}
+
+bool zstd_block_ok(uint32_t block)
+{
+  return block >= kZstdBlockMin && block <= kBlock && (block & (block - 1)) == 0;
+}
+uint32_t zstd_segments(size_t chunk_bytes, uint32_t block)
+{
+  return (uint32_t)((chunk_bytes + block - 1) / block);
+}
+uint64_t zstd_segment_stride(uint32_t block) { return block + kSegSlack; }
+size_t zstd_max_output_size(size_t chunk_bytes, uint32_t block)
+{
+  return (size_t)zstd_segments(chunk_bytes, block) * zstd_segment_stride(block);
+}
 size_t zstd_scratch_bytes(uint32_t workers) { return (size_t)workers * kWorkerScratch; }
 
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
-                       void* scratch, uint32_t workers, uint32_t* queue_counter,
+                       uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
                        hipStream_t stream)
 {
   (void)level;
   if (n == 0)
     return hipSuccess;
-  if (chunk_bytes == 0 || chunk_bytes > 0xFFFFFFFFull)
+  if (chunk_bytes == 0 || chunk_bytes > 0xFFFFFFFFull || !zstd_block_ok(block))
     return hipErrorInvalidValue;
-  const uint32_t nseg = zstd_segments(chunk_bytes);
+  const zcore::Predefined* pre = predefined_on_current_device();
+  if (!pre)
+    return hipErrorNotInitialized; // zstd_prepare() was not called on this device
+  const uint32_t nseg = zstd_segments(chunk_bytes, block);
   const uint64_t total = n * nseg;
   if (queue_counter) {
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(1), 0, stream, queue_counter);
     hipLaunchKernelGGL(zstd_k, dim3(workers), dim3(64), 0, stream, src, src_stride,
-                       (uint32_t)chunk_bytes, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, queue_counter);
+                       (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
+                       (uint8_t*)scratch, queue_counter, pre);
   } else {
     if (total > workers)
       return hipErrorInvalidValue; // one scratch slice per block in this mode
     hipLaunchKernelGGL(zstd_k, dim3((uint32_t)total), dim3(64), 0, stream, src, src_stride,
-                       (uint32_t)chunk_bytes, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, nullptr);
+                       (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
+                       (uint8_t*)scratch, nullptr, pre);
   }
   return hipGetLastError();
 }

@@ -458,6 +458,7 @@ struct HufWork
   uint8_t fse_buf[192];
   uint8_t spread[512];
   uint32_t cumul[66];
+  uint16_t blc[kHufMaxBits + 2];
   FseCTable ct;
 };
 
@@ -522,39 +523,57 @@ ZHD inline void huf_build_sorted(HufCode& h, const uint32_t* count, int n, HufWo
   for (int i = 0; i < n; ++i)
     maxlen = depth[i] > maxlen ? depth[i] : maxlen;
   if (maxlen > kHufMaxBits) {
-    // Clamp, then restore Kraft sum == 2^L: lengthen the longest codes below
-    // L (least frequent first) while over-full, then shorten the most
-    // frequent codes while under-full.
+    // Length limiting on the per-length counts (O(n + L^2), no per-symbol
+    // search): clamp to L, then move codes between lengths until the Kraft
+    // sum is exactly 2^L (complete code), and hand the lengths back out in
+    // count order, longest to the least frequent symbols.
     const int L = kHufMaxBits;
+    uint16_t* bl = ws.blc;
+    for (int b = 0; b <= L; ++b)
+      bl[b] = 0;
     int64_t kraft = 0;
     for (int i = 0; i < n; ++i) {
-      if (depth[i] > L)
-        depth[i] = (uint8_t)L;
-      kraft += (int64_t)1 << (L - depth[i]);
+      const int d = depth[i] > L ? L : depth[i];
+      bl[d]++;
+      kraft += (int64_t)1 << (L - d);
     }
     const int64_t full = (int64_t)1 << L;
     while (kraft > full) {
-      int best = -1;
-      for (int i = 0; i < n; ++i) // leaves ascending by count: first hit is least frequent
-        if (depth[i] < L && (best < 0 || depth[i] > depth[best]))
-          best = i;
-      kraft -= (int64_t)1 << (L - depth[best] - 1);
-      depth[best]++;
+      // lengthen codes at the longest length below L (smallest decrement)
+      int b = L - 1;
+      while (bl[b] == 0)
+        --b;
+      const int64_t step = (int64_t)1 << (L - b - 1);
+      int64_t k = (kraft - full + step - 1) / step;
+      if (k > bl[b])
+        k = bl[b];
+      bl[b] = (uint16_t)(bl[b] - k);
+      bl[b + 1] = (uint16_t)(bl[b + 1] + k);
+      kraft -= k * step;
     }
     while (kraft < full) {
-      int best = -1;
-      for (int i = n - 1; i >= 0; --i) // most frequent first
-        if (depth[i] > 1 && ((int64_t)1 << (L - depth[i])) <= full - kraft &&
-            (best < 0 || depth[i] < depth[best]))
-          best = i;
-      if (best < 0)
-        break;
-      kraft += (int64_t)1 << (L - depth[best]);
-      depth[best]--;
+      // shorten codes at the longest length whose gain still fits
+      int b = L;
+      while (b > 1 && (bl[b] == 0 || ((int64_t)1 << (L - b)) > full - kraft))
+        --b;
+      if (b <= 1)
+        break; // cannot happen for n >= 2
+      const int64_t step = (int64_t)1 << (L - b);
+      int64_t k = (full - kraft) / step;
+      if (k > bl[b])
+        k = bl[b];
+      bl[b] = (uint16_t)(bl[b] - k);
+      bl[b - 1] = (uint16_t)(bl[b - 1] + k);
+      kraft += k * step;
     }
+    int i = 0;
+    for (int b = L; b >= 1; --b)
+      for (int k = 0; k < bl[b]; ++k)
+        depth[i++] = (uint8_t)b;
     maxlen = 0;
-    for (int i = 0; i < n; ++i)
-      maxlen = depth[i] > maxlen ? depth[i] : maxlen;
+    for (int b = L; b >= 1 && !maxlen; --b)
+      if (bl[b])
+        maxlen = b;
   }
   for (int i = 0; i < n; ++i)
     h.len[sym[i]] = depth[i];

@@ -167,6 +167,10 @@ int chucky_device_synchronize(void);
 int chucky_synth_sensor_frames(void* d_dst, uint64_t frames, uint32_t height, uint32_t width,
                                uint64_t seed, double lambda, const float* blobs, uint32_t nblob);
 
+/* Seeded synthetic fill of n elements of dt in device memory: integers
+ * uniform over [0, int_range) (0 = every bit pattern), floats N(0,1) * 100. */
+int chucky_synth_fill(void* d_dst, uint64_t n, enum dtype dt, uint64_t seed, uint64_t int_range);
+
 #ifdef __cplusplus
 }
 #endif

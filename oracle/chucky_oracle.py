@@ -575,13 +575,34 @@ def write_stream(dims: List[ODim], dtype, data: np.ndarray, reduce_method=MEAN,
             return (acc / dt.type(count)).astype(dt)
         return acc.copy()
 
-    def flush_batch(n_epochs, mks, pool_base=None):
+    flushed = [False] * plan.K  # pool slot compressed by some flush so far
+
+    def flush_batch(n_epochs, mks, drain=False):
+        for e in range(n_epochs):
+            flushed[e] = True
         for lv in range(plan.nlod):
             L = plan.levels[lv]
             act = [e for e in range(n_epochs) if mks[e] & (1 << lv)]
             if not act:
                 continue
             chunks = []
+            if drain and (1 << lv) == plan.K:
+                # src/cpu/pipeline.c:159-180: the per-level batch LUT built at
+                # init (pool epoch (a+1)*period-1, src/cpu/pipeline.c:14-31)
+                # is reused whenever the active count equals the level's
+                # batch_active_count (K / period). For the one-epoch drain
+                # batch that holds exactly when period == K, so the drain
+                # delivers pool slot K-1 as last compressed -- the level's
+                # previous emission -- or zero-size (empty) chunks if no
+                # batch ever compressed that slot (comp_sizes is calloc'd,
+                # src/cpu/stream.c:70-71).
+                src = plan.K - 1
+                base = L.chunk_offset * CE
+                for j in range(L.chunks_per_epoch):
+                    chunks.append(pool[src, base + j * CE: base + (j + 1) * CE].tobytes()
+                                  if flushed[src] else b"")
+                deliver(plan, lv, shards[lv], chunks, 1, out)
+                continue
             for e in act:
                 base = L.chunk_offset * CE
                 for j in range(L.chunks_per_epoch):
@@ -618,7 +639,7 @@ def write_stream(dims: List[ODim], dtype, data: np.ndarray, reduce_method=MEAN,
                 lod.scatter(pool[0], lv, v)
                 drain |= 1 << lv
         if drain:
-            flush_batch(1, [drain])
+            flush_batch(1, [drain], drain=True)
     for lv in range(plan.nlod):
         if shards[lv].eis > 0:
             finalize(shards[lv], out, lv)

@@ -56,6 +56,9 @@ CASES = {
     "two_symbols": lambda: np.random.default_rng(2).integers(0, 2, 70_000, dtype=np.uint8).tobytes(),
     "tiny": lambda: b"abc",
     "just_over_block": lambda: np.random.default_rng(4).integers(0, 40, 131_073, dtype=np.uint8).tobytes(),
+    "geometric_lengths": lambda: np.random.default_rng(6).geometric(0.25, 200_000).clip(0, 255).astype(np.uint8).tobytes(),
+    "dominant_zero_rare_tail": lambda: np.where(np.random.default_rng(7).random(200_000) < 0.99, 0,
+                                                np.random.default_rng(8).integers(1, 256, 200_000)).astype(np.uint8).tobytes(),
     "skewed_257_symbols": lambda: np.random.default_rng(5).zipf(1.3, 90_000).clip(0, 255).astype(np.uint8).tobytes(),
 }
 

@@ -153,3 +153,36 @@ def test_shard_index_layout_closed_form():
     assert struct.unpack("<I", blob[-4:])[0] == O.crc32c(idx)
     ents = [struct.unpack_from("<QQ", idx, 16 * i) for i in range(4)]
     assert ents == [(0, 16), (16, 16), (32, 16), (48, 16)]
+
+
+# The drain of a dim-0 downsampled stream, for every level whose fold period
+# equals the batch length K, delivers pool slot K-1 as last compressed (the
+# reference reuses its init-time batch LUT, src/cpu/pipeline.c:159-180): the
+# previous emission, or empty chunks when no full batch ever ran.
+DRAIN_CASES = [(40, 4), (40, 8), (40, 2), (8, 4), (12, 4), (16, 4), (24, 8), (20, 4)]
+
+
+@needs_ref
+@pytest.mark.parametrize("rows,K", DRAIN_CASES)
+def test_drain_batch_lut_reuse_matches_reference(rows, K):
+    dims = [Dim(rows, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
+            Dim(20, 4, 0, downsample=True)]
+    cfg = ca.Config(dims, dtype="f32", epochs_per_batch=K, buffer_capacity_bytes=4096)
+    data = random_typed(rows * 24 * 20, "f32", seed=33)
+    want, _, _ = run_ref(cfg, data, pieces=3)
+    got, _ = O.write_stream([O.ODim(rows, 4, 2, True), O.ODim(24, 4, 0, True),
+                             O.ODim(20, 4, 0, True)], np.float32, data, K=K)
+    assert set(got) == set(want)
+    for k in want:
+        assert got[k] == want[k], f"shard {k}"
+
+
+def test_drain_without_full_batch_is_empty():
+    # two epochs, K=4: level 2 (period 4) drains into a slot no batch compressed
+    dims = [O.ODim(8, 4, 2, True), O.ODim(24, 4, 0, True), O.ODim(20, 4, 0, True)]
+    data = np.arange(8 * 24 * 20, dtype=np.float32)
+    out, _ = O.write_stream(dims, np.float32, data, K=4)
+    blob = out[(2, 0)]
+    assert len(blob) == 16 * 4 + 4  # index only
+    idx = blob[:-4]
+    assert all(struct.unpack_from("<QQ", idx, 16 * i) == (2**64 - 1, 2**64 - 1) for i in range(4))

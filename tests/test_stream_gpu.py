@@ -29,6 +29,28 @@ def _geom(cfg):
     return g, E, chunk_bytes
 
 
+def _where(got: bytes, want: bytes, cps_total: int, chunk_bytes: int, dt) -> str:
+    """Names the slots and the first elements where two CODEC_NONE shards differ."""
+    try:
+        gi, ri = parse_shard(got, cps_total), parse_shard(want, cps_total)
+    except AssertionError as e:
+        return f"unparseable ({e})"
+    if gi != ri:
+        return f"index differs: got {gi} want {ri}"
+    msgs = []
+    for slot, ent in enumerate(ri):
+        if ent is None:
+            continue
+        off, n = ent
+        a = np.frombuffer(got[off:off + n], dtype=dt)
+        b = np.frombuffer(want[off:off + n], dtype=dt)
+        bad = np.flatnonzero(a != b)
+        if bad.size:
+            msgs.append(f"slot {slot}: {bad.size}/{a.size} differ, first at {bad[:4].tolist()} "
+                        f"got {a[bad[:4]].tolist()} want {b[bad[:4]].tolist()}")
+    return "; ".join(msgs) or "data equal, bytes outside chunks differ"
+
+
 def _compare(cfg, data, pieces=1, device_input=False, pipeline=0):
     g, _, chunk_bytes = _geom(cfg)
     gs, gmeta, gcur = run_gpu(cfg, data, pieces=pieces, device_input=device_input,
@@ -40,7 +62,8 @@ def _compare(cfg, data, pieces=1, device_input=False, pipeline=0):
         assert gmeta[lv] == rmeta[lv], (lv, gmeta[lv], rmeta[lv])
     if cfg.codec == "none":
         for k in rs:
-            assert gs[k] == rs[k], f"shard {k} differs"
+            assert gs[k] == rs[k], f"shard {k} differs: " + _where(gs[k], rs[k], g[k[0]][0],
+                                                                   chunk_bytes, cfg.np_dtype)
         return gs
     cps = lambda lv: g[lv][0]  # noqa: E731
     for k in rs:
@@ -166,6 +189,16 @@ def test_batched_pipeline_matches_reference(name, device_input):
     make, dims, dtype, codec, extra = BATCHED_CASES[name]
     cfg = ca.Config(dims, dtype=dtype, codec=codec, **extra)
     _compare(cfg, make(), pieces=3, device_input=device_input, pipeline=ca.PIPELINE_BATCHED)
+
+
+@pytest.mark.parametrize("rows,K,codec", [(40, 4, "none"), (40, 8, "lz4"), (12, 4, "none"),
+                                          (24, 8, "zstd"), (40, 2, "none")])
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_drain_batch_lut_reuse(rows, K, codec, pipeline):
+    # levels whose fold period equals K drain pool slot K-1 as last compressed
+    cfg = ca.Config([Dim(rows, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
+                     Dim(20, 4, 0, downsample=True)], dtype="f32", codec=codec, epochs_per_batch=K)
+    _compare(cfg, random_typed(rows * 24 * 20, "f32", seed=33), pieces=3, pipeline=pipeline)
 
 
 def test_multiscale_dim0_zstd():
