@@ -88,7 +88,7 @@ def make_input(name, shape, seed=20260403):
     return ca.DeviceBuffer.from_host(vol), vol
 
 
-def run(name, frames, steps, warmup, pipeline, host_input):
+def run(name, frames, steps, warmup, pipeline, host_input, options=None):
     import chucky_amd as ca
     cfg, shape, desc = workload(name, frames)
     data, host = make_input(name, shape)
@@ -96,7 +96,7 @@ def run(name, frames, steps, warmup, pipeline, host_input):
     if host_input and host is None:
         host = data.to_host(nbytes).view(np.uint16)
     sink = ca.DiscardSink()
-    s = ca.GpuStream(cfg, sink, pipeline=pipeline)
+    s = ca.GpuStream(cfg, sink, pipeline=pipeline, **(options or {}))
 
     def one_step():
         assert s.reset(sink) == 0
@@ -281,6 +281,9 @@ def main():
     ap.add_argument("--host-input", action="store_true")
     ap.add_argument("--cpu-sample-frames", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--zstd-block", type=int, default=0, help="zstd block bytes (0 = autotuned)")
+    ap.add_argument("--gather-wg", type=int, default=0,
+                    help="workgroups of the gather into pinned host memory (0 = autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -296,7 +299,9 @@ def main():
     data = None
     for p in order:
         res, data, _ = run(args.workload, args.frames, args.steps, args.warmup,
-                           1 if p == "batched" else 0, args.host_input)
+                           1 if p == "batched" else 0, args.host_input,
+                           {"zstd_block_bytes": args.zstd_block,
+                            "host_gather_workgroups": args.gather_wg})
         runs[p] = res
     head = runs["batched" if "batched" in runs else order[0]]
     value, head_sum = summarize(head, args.steps)

@@ -220,7 +220,9 @@ class Server:
 class GpuStream:
     """tile_stream_gpu_* through its writer vtable."""
 
-    def __init__(self, cfg: Config, sink, pipeline: int = 0, workers: int = 0):
+    def __init__(self, cfg: Config, sink, pipeline: int = 0, workers: int = 0, **options):
+        """options: fields of struct tile_stream_gpu_options (zstd_block_bytes,
+        lz4_hash_log, host_gather_workgroups, device); 0 keeps the autotuned value."""
         self.cfg = cfg
         self._c = cfg.to_c()
         self.sink = sink
@@ -228,6 +230,8 @@ class GpuStream:
         lib.tile_stream_gpu_options_default(C.byref(opts))
         opts.pipeline = pipeline
         opts.compress_workers = workers
+        for k, v in options.items():
+            setattr(opts, k, v)
         self.h = lib.tile_stream_gpu_create_ex(C.byref(self._c), sink.shard_sink, C.byref(opts))
         if not self.h:
             raise RuntimeError("tile_stream_gpu_create failed: " + last_error())

@@ -89,7 +89,9 @@ __device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p)
   return (uint32_t)(((uint64_t)w[1] << 32 | w[0]) >> sh);
 }
 
-// grid.x = item, grid.y = segment (strided). Copies the segment's bytes.
+// Workgroups walk the (item, segment) pairs with a grid stride: a gather
+// into pinned host memory runs at PCIe speed, so it is given only a few CUs
+// and leaves the rest to the compressor and transpose running beside it.
 __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint32_t* __restrict__ gather,
                                                           const uint32_t* __restrict__ perm,
@@ -97,12 +99,11 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint64_t* __restrict__ offsets,
                                                           uint8_t* __restrict__ dst)
 {
-  const uint64_t item = blockIdx.x;
-  if (item >= n_items)
-    return;
-  const uint32_t chunk = gather[item];
-  const uint64_t base_out = offsets[perm[item]];
-  for (uint32_t seg = blockIdx.y; seg < src.nseg; seg += gridDim.y) {
+  const uint64_t pairs = n_items * src.nseg;
+  for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x) {
+    const uint64_t item = pr / src.nseg;
+    const uint32_t seg = (uint32_t)(pr - item * src.nseg);
+    const uint32_t chunk = gather[item];
     uint64_t seg_off = 0, nbytes;
     if (src.seg_sizes) {
       for (uint32_t k = 0; k < seg; ++k)
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
     if (nbytes == 0)
       continue;
     const uint8_t* in = src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride;
-    uint8_t* out = dst + base_out + seg_off;
+    uint8_t* out = dst + offsets[perm[item]] + seg_off;
     // head bytes up to a dword boundary of the destination
     const uint32_t head = (uint32_t)((4 - ((uintptr_t)out & 3)) & 3);
     const uint64_t h = head < nbytes ? head : nbytes;
@@ -152,13 +153,16 @@ hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gathe
 
 hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gather,
                                    const uint32_t* perm, uint64_t n_items, const uint64_t* offsets,
-                                   uint8_t* dst, hipStream_t stream)
+                                   uint8_t* dst, hipStream_t stream, uint32_t max_workgroups)
 {
   if (n_items == 0)
     return hipSuccess;
-  const uint32_t gy = src.nseg < 64 ? src.nseg : 64;
-  hipLaunchKernelGGL(aggregate_gather_k, dim3((uint32_t)n_items, gy), dim3(256), 0, stream, src,
-                     gather, perm, n_items, offsets, dst);
+  uint64_t wg = n_items * src.nseg;
+  const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
+  if (wg > cap)
+    wg = cap;
+  hipLaunchKernelGGL(aggregate_gather_k, dim3((uint32_t)wg), dim3(256), 0, stream, src, gather, perm,
+                     n_items, offsets, dst);
   return hipGetLastError();
 }
 

@@ -34,4 +34,13 @@ uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch)
   return kZstdBlockMin;
 }
 
+uint32_t tune_host_gather_workgroups()
+{
+  // PCIe-bound stores: a quarter of the CUs keeps the link busy
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return (uint32_t)(cus / 4 > 0 ? cus / 4 : 1);
+}
+
 } // namespace chucky

@@ -49,9 +49,11 @@ hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gathe
 
 // Copy every item's segments to dst + offsets[perm[i]] (dst may be device or
 // pinned host memory).
+// max_workgroups caps the grid (0 = one workgroup per (item, segment));
+// gathers into host memory use a small cap so they do not hold every CU.
 hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gather,
                                    const uint32_t* perm, uint64_t n_items, const uint64_t* offsets,
-                                   uint8_t* dst, hipStream_t stream);
+                                   uint8_t* dst, hipStream_t stream, uint32_t max_workgroups = 0);
 
 // ---- codecs ----
 

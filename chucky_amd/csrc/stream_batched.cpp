@@ -175,7 +175,7 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
     BCHECK(hipMemcpyAsync(H.h_offsets[lv], P.d_offsets[lv], (C + 1) * 8, hipMemcpyDeviceToHost, st));
     BCHECK(hipMemcpyAsync(H.h_sizes[lv], P.d_sizes[lv], C * 8, hipMemcpyDeviceToHost, st));
     BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
-                                   H.h_agg + H.level_base[lv], st));
+                                   H.h_agg + H.level_base[lv], st, s->host_gather_wg));
   }
   return 0;
 }

@@ -662,6 +662,8 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   HIPCHECK(hipStreamCreateWithFlags(&s->st_d2h, hipStreamNonBlocking));
 
   const uint32_t K = p.epochs_per_batch;
+  s->host_gather_wg = s->opts.host_gather_workgroups ? s->opts.host_gather_workgroups
+                                                     : tune_host_gather_workgroups();
   s->slot_compressed.assign(K, 0);
   const tile_stream_layout& L0 = p.levels[0].layout;
   s->chunk_stride_bytes = L0.chunk_stride * p.bpe;

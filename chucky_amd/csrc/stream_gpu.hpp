@@ -55,6 +55,7 @@ struct tile_stream_gpu
   void* d_zscratch = nullptr;
   uint32_t zworkers = 0;
   uint32_t zblock = chucky::kZstdBlockMax;
+  uint32_t host_gather_wg = 64;
   uint32_t* d_queue = nullptr;
   int lz4_hash_log = 12;
 
@@ -128,6 +129,7 @@ std::recursive_mutex& graph_mutex();
 void scatter_collect_ready(tile_stream_gpu* s);
 uint32_t compressor_slots();
 uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch);
+uint32_t tune_host_gather_workgroups();
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
 int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);

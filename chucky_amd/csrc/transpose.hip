@@ -152,14 +152,22 @@ hipError_t launch_transpose(void* dst, const void* src, uint64_t n, int bpe, uin
 {
   if (n == 0)
     return hipSuccess;
-  // Drop unit dims; the period of the lifted space is the product of shapes.
+  // Drop unit dims, and leading dims of stride 0 (the append dimension's
+  // chunk index: its coordinate never moves the destination, so only the
+  // index modulo the product of the remaining shapes -- one epoch -- matters,
+  // which keeps long streams on the 32-bit index path). The period of the
+  // lifted space is the product of the kept shapes.
   Space32 s32{};
   Space64 s64{};
   uint64_t period = 1;
   int r = 0;
+  bool leading = true;
   for (int d = 0; d < space.rank; ++d) {
     if (space.shape[d] == 1)
       continue;
+    if (leading && space.strides[d] == 0)
+      continue;
+    leading = false;
     s64.shape[r] = space.shape[d];
     s64.stride[r] = space.strides[d];
     ++r;

@@ -32,6 +32,8 @@ struct tile_stream_gpu_options
   uint32_t compress_workers; /* persistent compressor workgroups; 0 = autotuned */
   uint32_t lz4_hash_log;     /* LDS hash table log2 entries; 0 = autotuned */
   uint32_t zstd_block_bytes; /* zstd block size (<= 131072); 0 = autotuned */
+  uint32_t host_gather_workgroups; /* CU budget of the gather into pinned host memory
+                                      (batched pipeline); 0 = autotuned */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);
