@@ -85,6 +85,56 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const uint32_t u = __shfl_xor(v, off, kWave);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+// A 512-entry array spread over the wave's registers: entry i lives in lane
+// i % 64, register i / 64. Reads and writes take a wave-uniform index, so a
+// sequential algorithm over the array (one that lane 0 would run out of LDS)
+// runs out of registers at a few cycles per access.
+struct Dist512
+{
+  uint32_t r[8];
+};
+
+__device__ __forceinline__ uint32_t dist_get(const Dist512& a, uint32_t i)
+{
+  uint32_t v;
+  switch (i >> 6) {
+    case 0: v = a.r[0]; break;
+    case 1: v = a.r[1]; break;
+    case 2: v = a.r[2]; break;
+    case 3: v = a.r[3]; break;
+    case 4: v = a.r[4]; break;
+    case 5: v = a.r[5]; break;
+    case 6: v = a.r[6]; break;
+    default: v = a.r[7]; break;
+  }
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(i & 63));
+}
+
+__device__ __forceinline__ void dist_set(Dist512& a, uint32_t i, uint32_t v)
+{
+  if ((uint32_t)lane_id() != (i & 63))
+    return;
+  switch (i >> 6) {
+    case 0: a.r[0] = v; break;
+    case 1: a.r[1] = v; break;
+    case 2: a.r[2] = v; break;
+    case 3: a.r[3] = v; break;
+    case 4: a.r[4] = v; break;
+    case 5: a.r[5] = v; break;
+    case 6: a.r[6] = v; break;
+    default: a.r[7] = v; break;
+  }
+}
+
 // Number of leading equal bytes (0..4) of two little-endian dwords.
 __device__ __forceinline__ uint32_t eq_bytes(uint32_t a, uint32_t b)
 {

@@ -234,6 +234,180 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
   }
 }
 
+// Huffman code lengths and canonical values for the n >= 2 used symbols in
+// L.u.huf.sym[0..n) (ascending count, ties by symbol), built by the whole
+// wave: two-queue Huffman merge and the depth pass over register-resident
+// arrays, length limiting to kHufMaxBits on per-length counts (Kraft sum
+// restored exactly), lengths handed out in count order, canonical values by
+// wave ballots. Same code as zcore::huf_build_sorted produces on the host.
+__device__ void huf_build_wave(Lds& L, uint32_t n)
+{
+  const uint32_t lane = (uint32_t)lane_id();
+  constexpr uint32_t Lmax = zcore::kHufMaxBits;
+  Dist512 W, P;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t idx = (uint32_t)k * kWave + lane;
+    W.r[k] = (k < 4 && idx < n) ? L.hist[L.u.huf.sym[idx]] : 0u;
+    P.r[k] = 0;
+  }
+  uint32_t leaf = 0, node = n, next = n;
+  for (uint32_t k = 0; k + 1 < n; ++k) {
+    uint32_t pick0 = 0, pick1 = 0, wsum = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const bool has_leaf = leaf < n, has_node = node < next;
+      const uint32_t wl = has_leaf ? dist_get(W, leaf) : 0xFFFFFFFFu;
+      const uint32_t wn = has_node ? dist_get(W, node) : 0xFFFFFFFFu;
+      uint32_t pick;
+      if (has_leaf && (!has_node || wl <= wn)) {
+        pick = leaf++;
+        wsum += wl;
+      } else {
+        pick = node++;
+        wsum += wn;
+      }
+      if (c == 0)
+        pick0 = pick;
+      else
+        pick1 = pick;
+    }
+    dist_set(W, next, wsum);
+    dist_set(P, pick0, next);
+    dist_set(P, pick1, next);
+    ++next;
+  }
+  Dist512 D;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    D.r[k] = 0;
+  for (int i = (int)next - 2; i >= 0; --i)
+    dist_set(D, (uint32_t)i, dist_get(D, dist_get(P, (uint32_t)i)) + 1);
+
+  // leaf depths of this lane (leaves k*64 + lane, k < 4)
+  uint32_t len[4];
+  uint32_t maxd = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t idx = (uint32_t)k * kWave + lane;
+    len[k] = idx < n ? D.r[k] : 0u;
+    maxd = len[k] > maxd ? len[k] : maxd;
+  }
+  maxd = wave_max(maxd);
+  if (maxd > Lmax) {
+    // per-length code counts live in lanes: lane b holds the count of length b
+    uint32_t blv = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t d = len[k] > Lmax ? Lmax : len[k];
+#pragma unroll
+      for (uint32_t b = 1; b <= Lmax; ++b) {
+        const uint32_t c = (uint32_t)__popcll(wave_ballot(len[k] != 0 && d == b));
+        if (lane == b)
+          blv += c;
+      }
+    }
+    const int64_t full = (int64_t)1 << Lmax;
+    int64_t kraft = (int64_t)wave_sum(lane >= 1 && lane <= Lmax ? blv << (Lmax - lane) : 0u);
+    while (kraft > full) {
+      // lengthen codes at the longest length below Lmax (smallest decrement)
+      const uint64_t nz = wave_ballot(lane >= 1 && lane < Lmax && blv != 0);
+      const uint32_t b = 63u - (uint32_t)__clzll(nz);
+      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
+      const int64_t step = (int64_t)1 << (Lmax - b - 1);
+      int64_t k = (kraft - full + step - 1) / step;
+      if (k > (int64_t)cnt)
+        k = cnt;
+      if (lane == b)
+        blv -= (uint32_t)k;
+      if (lane == b + 1)
+        blv += (uint32_t)k;
+      kraft -= k * step;
+    }
+    while (kraft < full) {
+      // shorten codes at the longest length whose gain still fits
+      const int64_t gap = full - kraft;
+      const uint64_t ok = wave_ballot(lane > 1 && lane <= Lmax && blv != 0 &&
+                                      ((int64_t)1 << (Lmax - lane)) <= gap);
+      if (!ok)
+        break;
+      const uint32_t b = 63u - (uint32_t)__clzll(ok);
+      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
+      const int64_t step = (int64_t)1 << (Lmax - b);
+      int64_t k = gap / step;
+      if (k > (int64_t)cnt)
+        k = cnt;
+      if (lane == b)
+        blv -= (uint32_t)k;
+      if (lane == b - 1)
+        blv += (uint32_t)k;
+      kraft += k * step;
+    }
+    // longest codes to the least frequent leaves: leaf ranks [above, above +
+    // count) of length b, with `above` the number of codes longer than b
+    uint32_t above = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      len[k] = 0;
+    for (uint32_t b = Lmax; b >= 1; --b) {
+      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t idx = (uint32_t)k * kWave + lane;
+        if (idx < n && idx >= above && idx < above + cnt)
+          len[k] = b;
+      }
+      above += cnt;
+    }
+    const uint64_t nz = wave_ballot(lane >= 1 && lane <= Lmax && blv != 0);
+    maxd = 63u - (uint32_t)__clzll(nz);
+  }
+  // scatter lengths to symbols
+  for (uint32_t s = lane; s < 256; s += kWave)
+    L.code.len[s] = 0;
+  __syncthreads();
+  uint32_t maxsym = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t idx = (uint32_t)k * kWave + lane;
+    if (idx < n) {
+      const uint32_t s = L.u.huf.sym[idx];
+      L.code.len[s] = (uint8_t)len[k];
+      maxsym = s > maxsym ? s : maxsym;
+    }
+  }
+  maxsym = wave_max(maxsym);
+  __syncthreads();
+  // canonical values: per length, consecutive in symbol order, starting
+  // values from the longest length down
+  uint32_t sl[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    sl[k] = L.code.len[(uint32_t)k * kWave + lane];
+  uint32_t val[4] = { 0, 0, 0, 0 };
+  uint32_t minv = 0;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (uint32_t l = maxd; l >= 1; --l) {
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t m = wave_ballot(sl[k] == l);
+      if (sl[k] == l)
+        val[k] = minv + rank + (uint32_t)__popcll(m & below);
+      rank += (uint32_t)__popcll(m);
+    }
+    minv = (minv + rank) >> 1;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    L.code.val[(uint32_t)k * kWave + lane] = (uint16_t)val[k];
+  if (lane == 0) {
+    L.code.max_bits = (int)maxd;
+    L.code.max_sym = (int)maxsym;
+  }
+  __syncthreads();
+}
+
 // Compresses one block; returns bytes written at out (header included).
 __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
                                    uint8_t* lits, Seq* seqs, uint8_t* seqout,
@@ -347,10 +521,9 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
     for (uint32_t i = lane; i < used; i += kWave)
       L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
     __syncthreads();
-    if (lane == 0) {
-      zcore::huf_build_sorted(L.code, L.hist, (int)used, L.u.huf);
+    huf_build_wave(L, used);
+    if (lane == 0)
       L.tree_size = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
-    }
     __syncthreads();
     uint32_t bits_local = 0;
     for (int s = lane; s < 256; s += kWave)

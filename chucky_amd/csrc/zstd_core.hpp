@@ -243,16 +243,25 @@ ZHD inline size_t fse_encode_two_state(uint8_t* dst, const uint8_t* src, size_t 
     return 0;
   BitW b;
   bw_init(b, dst);
-  FseState st[2];
+  // state of the even and of the odd positions (named, not an indexed pair:
+  // on the device an indexed local array would live in scratch memory)
+  FseState se, so;
   long i = (long)n - 1;
-  fse_init2(st[i & 1], ct, src[i]);
-  --i;
-  fse_init2(st[i & 1], ct, src[i]);
-  --i;
-  for (; i >= 0; --i)
-    fse_encode(b, st[i & 1], ct, src[i]);
-  fse_flush_state(b, st[1], ct);
-  fse_flush_state(b, st[0], ct);
+  if (i & 1) {
+    fse_init2(so, ct, src[i]);
+    fse_init2(se, ct, src[i - 1]);
+  } else {
+    fse_init2(se, ct, src[i]);
+    fse_init2(so, ct, src[i - 1]);
+  }
+  for (i -= 2; i >= 0; --i) {
+    if (i & 1)
+      fse_encode(b, so, ct, src[i]);
+    else
+      fse_encode(b, se, ct, src[i]);
+  }
+  fse_flush_state(b, so, ct);
+  fse_flush_state(b, se, ct);
   return bw_close(b);
 }
 
@@ -459,6 +468,8 @@ struct HufWork
   uint8_t spread[512];
   uint32_t cumul[66];
   uint16_t blc[kHufMaxBits + 2];
+  uint32_t wcount[16]; // weight histogram of the tree description
+  int16_t wnorm[16];   // its normalized counts
   FseCTable ct;
 };
 
@@ -604,7 +615,9 @@ ZHD inline size_t huf_write_tree(uint8_t* out, const HufCode& h, HufWork& ws)
   size_t fse_size = 0;
   uint8_t* fse_buf = ws.fse_buf;
   if (nw > 2) {
-    uint32_t cnt[16] = {};
+    uint32_t* cnt = ws.wcount;
+    for (int w = 0; w < 16; ++w)
+      cnt[w] = 0;
     int maxw = 0, distinct = 0;
     for (int s = 0; s < nw; ++s) {
       if (!cnt[wt[s]]++)
@@ -612,7 +625,7 @@ ZHD inline size_t huf_write_tree(uint8_t* out, const HufCode& h, HufWork& ws)
       maxw = wt[s] > maxw ? wt[s] : maxw;
     }
     if (distinct >= 2) {
-      int16_t norm[16];
+      int16_t* norm = ws.wnorm;
       fse_normalize(norm, cnt, maxw, (uint32_t)nw, 6);
       FseCTable& ct = ws.ct;
       fse_build_ctable(ct, norm, maxw, 6, ws.spread, ws.cumul);

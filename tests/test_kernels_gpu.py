@@ -67,6 +67,14 @@ DATA = {
     "random": lambda: np.random.default_rng(1).integers(0, 256, 200_000, dtype=np.uint8).tobytes(),
     "sensor_like": lambda: np.clip(np.random.default_rng(2).poisson(100, 150_000), 0, 4095)
     .astype(np.uint16).tobytes(),
+    # Huffman depths beyond 11 bits: length limiting on the device
+    "geometric": lambda: np.random.default_rng(6).geometric(0.25, 400_000).clip(0, 255)
+    .astype(np.uint8).tobytes(),
+    "dominant_rare_tail": lambda: np.where(np.random.default_rng(7).random(400_000) < 0.99, 0,
+                                           np.random.default_rng(8).integers(1, 256, 400_000))
+    .astype(np.uint8).tobytes(),
+    "two_symbols": lambda: np.random.default_rng(9).integers(0, 2, 300_000, dtype=np.uint8).tobytes(),
+    "all_256": lambda: np.tile(np.arange(256, dtype=np.uint8), 1200).tobytes(),
 }
 
 
