@@ -111,17 +111,20 @@ def run(name, frames, steps, warmup, pipeline, host_input, options=None):
     b0 = sink.bytes
     m0 = snapshot(s)
     lat = []
+    step_ms = []
     ca.lib.chucky_device_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
+        ts = time.perf_counter()
         one_step()
+        step_ms.append(round((time.perf_counter() - ts) * 1e3, 2))
         lat.extend(s.latencies())
     ca.lib.chucky_device_synchronize()
     dt = time.perf_counter() - t0
     m1 = snapshot(s)
     res = dict(seconds=dt, bytes=nbytes, out_bytes=(sink.bytes - b0) / max(steps, 1), desc=desc,
                metrics={k: tuple(m1[k][i] - m0[k][i] for i in range(3)) for k in m1},
-               latencies=lat, ms_per_step=dt * 1e3 / max(steps, 1))
+               latencies=lat, ms_per_step=dt * 1e3 / max(steps, 1), step_ms=step_ms)
     s.close()
     return res, data, cfg
 
@@ -164,9 +167,13 @@ def summarize(res, steps):
     comp_ms, comp_n, comp_in = m["compress"]
     lat = sorted(res["latencies"])
     pct = (lambda q: round(lat[min(len(lat) - 1, int(q * len(lat)))] * 1e3, 3)) if lat else (lambda q: None)
+    stages = {k: {"ms": round(v[0] / steps, 3), "launches": v[1] // max(steps, 1),
+                  "GBps": round(v[2] / v[0] / 1e6, 2) if v[0] > 0 else None}
+              for k, v in m.items() if v[1]}
     return value, dict(GBps=round(value, 3), ms_per_step=round(res["ms_per_step"], 3),
                        ratio=round(res["bytes"] / max(res["out_bytes"], 1), 4),
-                       p50_epoch_latency_ms=pct(0.50), p99_epoch_latency_ms=pct(0.99))
+                       p50_epoch_latency_ms=pct(0.50), p99_epoch_latency_ms=pct(0.99),
+                       step_ms=res.get("step_ms"), stages_per_step=stages)
 
 
 SERVING_SHAPES = [("u8", (2048, 512, 512)), ("u16", (256, 2048, 2048)), ("f32", (256, 256, 256))]
@@ -284,6 +291,8 @@ def main():
     ap.add_argument("--zstd-block", type=int, default=0, help="zstd block bytes (0 = autotuned)")
     ap.add_argument("--gather-wg", type=int, default=0,
                     help="workgroups of the gather into pinned host memory (0 = autotuned)")
+    ap.add_argument("--batched-d2h", type=int, default=0,
+                    help="batched pipeline D2H: 0 = device gather + DMA, 1 = gather kernel into host")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -301,7 +310,8 @@ def main():
         res, data, _ = run(args.workload, args.frames, args.steps, args.warmup,
                            1 if p == "batched" else 0, args.host_input,
                            {"zstd_block_bytes": args.zstd_block,
-                            "host_gather_workgroups": args.gather_wg})
+                            "host_gather_workgroups": args.gather_wg,
+                            "batched_d2h": args.batched_d2h})
         runs[p] = res
     head = runs["batched" if "batched" in runs else order[0]]
     value, head_sum = summarize(head, args.steps)

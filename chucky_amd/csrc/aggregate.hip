@@ -89,9 +89,36 @@ __device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p)
   return (uint32_t)(((uint64_t)w[1] << 32 | w[0]) >> sh);
 }
 
-// Workgroups walk the (item, segment) pairs with a grid stride: a gather
-// into pinned host memory runs at PCIe speed, so it is given only a few CUs
-// and leaves the rest to the compressor and transpose running beside it.
+// Workgroup-cooperative copy of n bytes to any destination alignment: byte
+// head to a dword boundary of dst, dword body (aligned or funnel-shifted
+// source loads), byte tail.
+__device__ __forceinline__ void block_copy(uint8_t* out, const uint8_t* in, uint64_t nbytes)
+{
+  const uint32_t head = (uint32_t)((4 - ((uintptr_t)out & 3)) & 3);
+  const uint64_t h = head < nbytes ? head : nbytes;
+  if (threadIdx.x < h)
+    out[threadIdx.x] = in[threadIdx.x];
+  const uint64_t body = (nbytes - h) / 4;
+  uint32_t* ow = (uint32_t*)(out + h);
+  const uint8_t* ib = in + h;
+  if (((uintptr_t)ib & 3) == 0) {
+    const uint32_t* iw = (const uint32_t*)ib;
+    for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
+      ow[w] = iw[w];
+  } else {
+    for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
+      ow[w] = load_u32_unaligned(ib + 4 * w);
+  }
+  const uint64_t tail0 = h + body * 4;
+  if (threadIdx.x < nbytes - tail0)
+    out[tail0 + threadIdx.x] = in[tail0 + threadIdx.x];
+}
+
+// Workgroups take items (compressed chunks) with a grid stride and copy their
+// segments in order with a running offset; constant-size sources (CODEC_NONE
+// chunks, split into fixed pieces) are walked as (item, piece) pairs instead.
+// A gather into pinned host memory runs at PCIe speed, so it is given only a
+// few CUs and leaves the rest to the compressor and transpose beside it.
 __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint32_t* __restrict__ gather,
                                                           const uint32_t* __restrict__ perm,
@@ -99,43 +126,32 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint64_t* __restrict__ offsets,
                                                           uint8_t* __restrict__ dst)
 {
+  if (src.seg_sizes) {
+    for (uint64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+      const uint32_t chunk = gather[item];
+      uint8_t* out = dst + offsets[perm[item]];
+      const uint8_t* in = src.base + (uint64_t)chunk * src.chunk_stride;
+      const uint32_t* sz = src.seg_sizes + (uint64_t)chunk * src.nseg;
+      uint64_t off = 0;
+      for (uint32_t seg = 0; seg < src.nseg; ++seg) {
+        const uint32_t nb = sz[seg];
+        block_copy(out + off, in + (uint64_t)seg * src.seg_stride, nb);
+        off += nb;
+      }
+    }
+    return;
+  }
   const uint64_t pairs = n_items * src.nseg;
   for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x) {
     const uint64_t item = pr / src.nseg;
     const uint32_t seg = (uint32_t)(pr - item * src.nseg);
-    const uint32_t chunk = gather[item];
-    uint64_t seg_off = 0, nbytes;
-    if (src.seg_sizes) {
-      for (uint32_t k = 0; k < seg; ++k)
-        seg_off += src.seg_sizes[(uint64_t)chunk * src.nseg + k];
-      nbytes = src.seg_sizes[(uint64_t)chunk * src.nseg + seg];
-    } else {
-      seg_off = (uint64_t)seg * src.const_size;
-      nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
-    }
-    if (nbytes == 0)
+    const uint64_t seg_off = (uint64_t)seg * src.const_size;
+    const uint64_t nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
+    if (nbytes == 0 || seg_off >= src.const_total)
       continue;
-    const uint8_t* in = src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride;
-    uint8_t* out = dst + offsets[perm[item]] + seg_off;
-    // head bytes up to a dword boundary of the destination
-    const uint32_t head = (uint32_t)((4 - ((uintptr_t)out & 3)) & 3);
-    const uint64_t h = head < nbytes ? head : nbytes;
-    if (threadIdx.x < h)
-      out[threadIdx.x] = in[threadIdx.x];
-    const uint64_t body = (nbytes - h) / 4;
-    uint32_t* ow = (uint32_t*)(out + h);
-    const uint8_t* ib = in + h;
-    if (((uintptr_t)ib & 3) == 0) {
-      const uint32_t* iw = (const uint32_t*)ib;
-      for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
-        ow[w] = iw[w];
-    } else {
-      for (uint64_t w = threadIdx.x; w < body; w += blockDim.x)
-        ow[w] = load_u32_unaligned(ib + 4 * w);
-    }
-    const uint64_t tail0 = h + body * 4;
-    if (threadIdx.x < nbytes - tail0)
-      out[tail0 + threadIdx.x] = in[tail0 + threadIdx.x];
+    const uint32_t chunk = gather[item];
+    block_copy(dst + offsets[perm[item]] + seg_off,
+               src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nbytes);
   }
 }
 
@@ -157,7 +173,7 @@ hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gat
 {
   if (n_items == 0)
     return hipSuccess;
-  uint64_t wg = n_items * src.nseg;
+  uint64_t wg = src.seg_sizes ? n_items : n_items * src.nseg;
   const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
   if (wg > cap)
     wg = cap;

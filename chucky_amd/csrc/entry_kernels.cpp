@@ -47,7 +47,7 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
     goto done;
   } else if (codec == CODEC_LZ4_NON_STANDARD) {
     if (launch_lz4((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
-                   seg, 12, st) != hipSuccess)
+                   seg, 12, 0, nullptr, st) != hipSuccess)
       goto done;
   } else if (codec == CODEC_ZSTD) {
     const uint32_t workers = (uint32_t)(n * nseg < 2048 ? n * nseg : 2048);
@@ -57,7 +57,7 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
         hipMalloc((void**)&queue, 256) != hipSuccess)
       goto done;
     if (launch_zstd((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
-                    seg, level, kZstdBlockMax, scratch, workers, queue, st) != hipSuccess)
+                    seg, level, kZstdBlockMax, scratch, workers, queue, nullptr, st) != hipSuccess)
       goto done;
   } else {
     set_error("chucky_gpu_compress: unsupported codec");

@@ -68,10 +68,13 @@ size_t zstd_max_output_size(size_t chunk_bytes, uint32_t block);
 uint32_t zstd_segments(size_t chunk_bytes, uint32_t block);
 uint64_t zstd_segment_stride(uint32_t block);
 
-// LZ4 raw block per chunk: one wavefront per chunk.
+// LZ4 raw block per chunk: one wavefront per chunk, or (seg_bytes < chunk
+// bytes) one wavefront per segment of seg_bytes plus a per-chunk stitch,
+// using `scratch` of lz4_scratch_bytes(chunk_bytes, n, seg_bytes).
+size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes);
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                       uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
-                      hipStream_t stream);
+                      uint32_t seg_bytes, void* scratch, hipStream_t stream);
 
 // zstd frame per chunk, split into independent blocks (segments) of `block` bytes.
 // Uploads the predefined FSE tables to the current device (once per device;
@@ -82,6 +85,11 @@ size_t zstd_scratch_bytes(uint32_t workers);
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
                        uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
-                       hipStream_t stream);
+                       void* tables, hipStream_t stream);
+// Chunk-table mode (tables != nullptr, zstd_table_bytes(n) of device memory):
+// one Huffman table per chunk, built from the chunk's byte histogram by a
+// workgroup per chunk, carried by a small first block of the frame and reused
+// by every block as treeless literals; otherwise every block builds its own.
+size_t zstd_table_bytes(uint64_t n);
 
 } // namespace chucky

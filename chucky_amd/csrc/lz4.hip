@@ -148,6 +148,176 @@ __global__ __launch_bounds__(64) void lz4_chunk_k(const uint8_t* __restrict__ sr
     sizes[chunk] = op;
 }
 
+struct SegRec
+{
+  uint32_t nseq; // sequences with a match
+  uint32_t ll0;  // literals of the first sequence inside the segment
+  uint32_t tok0; // match-length nibble of the first sequence
+  uint32_t tail; // trailing literals (carried into the next segment)
+  uint32_t body; // bytes written to scratch
+};
+
+// Writes a sequence's match part: offset, match-length extension.
+__device__ __forceinline__ uint32_t emit_match_part(uint8_t* out, uint32_t op, uint32_t mlen,
+                                                    uint32_t offset)
+{
+  const uint32_t ml4 = mlen - 4;
+  if (lane_id() == 0) {
+    out[op] = (uint8_t)offset;
+    out[op + 1] = (uint8_t)(offset >> 8);
+    if (ml4 >= 15)
+      put_len_ext(out, op + 2, ml4);
+  }
+  return op + 2 + (ml4 >= 15 ? 1 + (ml4 - 15) / 255 : 0);
+}
+
+template<int HASH_LOG>
+__global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ src,
+                                                    uint64_t src_stride, uint32_t n,
+                                                    uint32_t seg_len, uint32_t nseg,
+                                                    uint64_t total, uint8_t* __restrict__ tmp,
+                                                    uint64_t seg_bound, SegRec* __restrict__ recs)
+{
+  __shared__ uint32_t table[1 << HASH_LOG]; // position + 1, 0 = empty
+  const uint64_t item = blockIdx.x;
+  if (item >= total)
+    return;
+  const int lane = lane_id();
+  const uint64_t chunk = item / nseg;
+  const uint32_t seg = (uint32_t)(item - chunk * nseg);
+  const uint8_t* in = src + chunk * src_stride;
+  uint8_t* out = tmp + item * seg_bound;
+  const uint32_t s0 = seg * seg_len;
+  const uint32_t s1 = n - s0 < seg_len ? n : s0 + seg_len;
+  for (int i = lane; i < (1 << HASH_LOG); i += kWave)
+    table[i] = 0;
+  __syncthreads();
+
+  uint32_t op = 0, anchor = s0, nseq = 0, ll0 = 0, tok0 = 0;
+  // block-end rules against the chunk end; keys and matches stay in [s0, s1)
+  const uint32_t last_start = n < kMfLimit ? 0 : (s1 - 4 < n - kMfLimit ? s1 - 4 : n - kMfLimit);
+  const uint32_t match_end = s1 < n - kLastLiterals ? s1 : n - kLastLiterals;
+  if (n > kMfLimit && s1 - s0 >= 8 && last_start >= s0) {
+    uint32_t pos = s0;
+    while (pos <= last_start) {
+      const uint32_t stride = 1 + ((pos - anchor) >> 10);
+      const uint32_t p = pos + (uint32_t)lane * stride;
+      const bool active = p <= last_start;
+      uint32_t v = 0, h = 0, cand = 0;
+      if (active) {
+        v = ld32u(in + p);
+        h = lz4_hash(v, HASH_LOG);
+        cand = table[h];
+      }
+      bool hit = false;
+      if (active && cand != 0) {
+        const uint32_t c = cand - 1;
+        hit = c < p && (p - c) <= kMaxOffset && ld32u(in + c) == v;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (active)
+        table[h] = p + 1;
+      __builtin_amdgcn_wave_barrier();
+      const uint64_t mask = wave_ballot(hit);
+      if (mask == 0) {
+        pos += kWave * stride;
+        continue;
+      }
+      const int first = __ffsll((unsigned long long)mask) - 1;
+      uint32_t mp = pos + (uint32_t)first * stride;
+      uint32_t mc = wave_bcast(cand, first) - 1;
+      uint32_t len = wave_match_extend(in, mp, mc, 4, match_end);
+      while (mp > anchor && mc > 0 && (mp - mc) <= kMaxOffset && in[mp - 1] == in[mc - 1]) {
+        --mp;
+        --mc;
+        ++len;
+      }
+      if (nseq == 0) {
+        ll0 = mp - s0;
+        tok0 = len - 4 >= 15 ? 15 : len - 4;
+        op = emit_match_part(out, op, len, mp - mc);
+      } else {
+        op = emit(out, op, in, anchor, mp - anchor, true, len, mp - mc);
+      }
+      ++nseq;
+      pos = mp + len;
+      anchor = pos;
+      if (lane == 0 && pos - 2 <= last_start) {
+        const uint32_t q = pos - 2;
+        table[lz4_hash(ld32u(in + q), HASH_LOG)] = q + 1;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (lane == 0)
+    recs[item] = SegRec{ nseq, ll0, tok0, s1 - anchor, op };
+}
+
+// Workgroup-cooperative copy (any alignment): byte head to a dword boundary of
+// dst, dword body from funnel-shifted source, byte tail.
+__device__ __forceinline__ void block_copy(uint8_t* dst, const uint8_t* src, uint32_t n)
+{
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  const uint32_t head = (uint32_t)((4 - ((uintptr_t)dst & 3)) & 3);
+  const uint32_t h = head < n ? head : n;
+  if (t < h)
+    dst[t] = src[t];
+  const uint32_t body = (n - h) >> 2;
+  uint32_t* dw = (uint32_t*)(dst + h);
+  const uint8_t* s = src + h;
+  for (uint32_t w = t; w < body; w += nt)
+    dw[w] = ld32u(s + 4 * w);
+  const uint32_t t0 = h + body * 4;
+  if (t < n - t0)
+    dst[t0 + t] = src[t0 + t];
+}
+
+__device__ __forceinline__ uint32_t put_token(uint8_t* out, uint32_t op, uint32_t lit, uint32_t nib)
+{
+  if (threadIdx.x == 0) {
+    out[op] = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | nib);
+    if (lit >= 15)
+      put_len_ext(out, op + 1, lit);
+  }
+  return op + 1 + (lit >= 15 ? 1 + (lit - 15) / 255 : 0);
+}
+
+__global__ __launch_bounds__(256) void lz4_stitch_k(const uint8_t* __restrict__ src,
+                                                    uint64_t src_stride, uint32_t n,
+                                                    uint32_t seg_len, uint32_t nseg,
+                                                    uint64_t nchunks, const uint8_t* __restrict__ tmp,
+                                                    uint64_t seg_bound, const SegRec* __restrict__ recs,
+                                                    uint8_t* __restrict__ dst, uint64_t dst_stride,
+                                                    uint32_t* __restrict__ sizes)
+{
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= nchunks)
+    return;
+  const uint8_t* in = src + chunk * src_stride;
+  uint8_t* out = dst + chunk * dst_stride;
+  uint32_t op = 0, carry = 0; // carry: input position where pending literals start
+  for (uint32_t seg = 0; seg < nseg; ++seg) {
+    const SegRec r = recs[chunk * nseg + seg];
+    if (r.nseq == 0)
+      continue;
+    const uint32_t s0 = seg * seg_len;
+    const uint32_t lit = s0 + r.ll0 - carry;
+    op = put_token(out, op, lit, r.tok0);
+    block_copy(out + op, in + carry, lit);
+    op += lit;
+    block_copy(out + op, tmp + (chunk * nseg + seg) * seg_bound, r.body);
+    op += r.body;
+    const uint32_t s1 = n - s0 < seg_len ? n : s0 + seg_len;
+    carry = s1 - r.tail;
+  }
+  const uint32_t lit = n - carry;
+  op = put_token(out, op, lit, 0);
+  block_copy(out + op, in + carry, lit);
+  op += lit;
+  if (threadIdx.x == 0)
+    sizes[chunk] = op;
+}
+
 } // namespace
 
 size_t lz4_max_output_size(size_t chunk_bytes)
@@ -156,32 +326,61 @@ size_t lz4_max_output_size(size_t chunk_bytes)
   return chunk_bytes + chunk_bytes / 255 + 16;
 }
 
+uint64_t lz4_segment_bound(uint32_t seg_bytes) { return (lz4_max_output_size(seg_bytes) + 15) & ~15ull; }
+
+size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes)
+{
+  if (!seg_bytes || seg_bytes >= chunk_bytes)
+    return 0;
+  const uint64_t nseg = (chunk_bytes + seg_bytes - 1) / seg_bytes;
+  return n * nseg * (lz4_segment_bound(seg_bytes) + sizeof(SegRec));
+}
+
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                       uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
-                      hipStream_t stream)
+                      uint32_t seg_bytes, void* scratch, hipStream_t stream)
 {
   if (n == 0)
     return hipSuccess;
   if (chunk_bytes > 0x7E000000ull)
     return hipErrorInvalidValue; // LZ4 block size limit
-  switch (hash_log) {
-    case 10:
-      hipLaunchKernelGGL(lz4_chunk_k<10>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
-                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
-      break;
-    case 11:
-      hipLaunchKernelGGL(lz4_chunk_k<11>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
-                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
-      break;
-    case 13:
-      hipLaunchKernelGGL(lz4_chunk_k<13>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
-                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
-      break;
-    default:
-      hipLaunchKernelGGL(lz4_chunk_k<12>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,
-                         (uint32_t)chunk_bytes, n, dst, dst_stride, sizes);
-      break;
+  const bool segmented = seg_bytes && seg_bytes < chunk_bytes;
+  if (!segmented) {
+#define LZ4_CHUNK(H)                                                                          \
+  hipLaunchKernelGGL(lz4_chunk_k<H>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,   \
+                     (uint32_t)chunk_bytes, n, dst, dst_stride, sizes)
+    switch (hash_log) {
+      case 10: LZ4_CHUNK(10); break;
+      case 11: LZ4_CHUNK(11); break;
+      case 13: LZ4_CHUNK(13); break;
+      default: LZ4_CHUNK(12); break;
+    }
+#undef LZ4_CHUNK
+    return hipGetLastError();
   }
+  if (!scratch || seg_bytes < 1024)
+    return hipErrorInvalidValue;
+  const uint32_t nseg = (uint32_t)((chunk_bytes + seg_bytes - 1) / seg_bytes);
+  const uint64_t total = n * nseg;
+  const uint64_t bound = lz4_segment_bound(seg_bytes);
+  uint8_t* tmp = (uint8_t*)scratch;
+  SegRec* recs = (SegRec*)(tmp + total * bound);
+#define LZ4_SEG(H)                                                                            \
+  hipLaunchKernelGGL(lz4_segment_k<H>, dim3((uint32_t)total), dim3(64), 0, stream, src,         \
+                     src_stride, (uint32_t)chunk_bytes, seg_bytes, nseg, total, tmp, bound, recs)
+  switch (hash_log) {
+    case 10: LZ4_SEG(10); break;
+    case 11: LZ4_SEG(11); break;
+    case 13: LZ4_SEG(13); break;
+    default: LZ4_SEG(12); break;
+  }
+#undef LZ4_SEG
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(lz4_stitch_k, dim3((uint32_t)n), dim3(256), 0, stream, src, src_stride,
+                     (uint32_t)chunk_bytes, seg_bytes, nseg, n, tmp, bound, recs, dst, dst_stride,
+                     sizes);
   return hipGetLastError();
 }
 

@@ -5,9 +5,15 @@
 //   compute queue:  scatter/LOD(e) -> [graph: compress all chunks of epoch e]
 //   aggregate queue: wait(compress e) -> [graph: per active level, size plan +
 //                    exclusive scan, offsets to pinned memory, gather of the
-//                    compressed bytes straight into a pinned host slot]
-//   host:           delivers epochs in order once their aggregate event has
-//                   completed (sink writes, index entries, shard finalisation)
+//                    compressed bytes into a device staging slot]
+//   copy queue:      one DMA copy per active level of exactly the gathered
+//                    bytes into a pinned host slot, issued by the host as soon
+//                    as the epoch's offsets are known (the copy engines move
+//                    the data at link speed without taking CUs from the
+//                    compressor; a gather kernel writing host memory directly
+//                    is kept as an option)
+//   host:           delivers epochs in order once their copy has completed
+//                   (sink writes, index entries, shard finalisation)
 //
 // The compress graph of epoch e+1 runs while the gather of epoch e streams
 // over PCIe. Epochs rotate through the K pool slots of the plan and through R
@@ -41,10 +47,13 @@ namespace {
 struct BatchedState
 {
   hipStream_t st_agg = nullptr;
+  hipStream_t st_copy = nullptr;
+  bool dma = true; // gathered bytes reach host memory by DMA copies
   uint32_t R = 0; // host slots
   struct HostSlot
   {
     uint8_t* h_agg = nullptr;
+    uint8_t* d_stage = nullptr; // device staging of the gathered bytes (DMA mode)
     std::vector<uint64_t> level_base; // byte offset of each level's bytes
     std::vector<uint64_t*> h_offsets, h_sizes;
     std::vector<io_event> fence;
@@ -65,7 +74,8 @@ struct BatchedState
   {
     uint64_t epoch;
     uint32_t pool_slot, host_slot, mask;
-    hipEvent_t done;
+    hipEvent_t done;   // aggregate graph finished
+    hipEvent_t copied; // DMA copies finished (DMA mode), else null
     hipEvent_t c0, c1; // compress graph start / end (timing events)
     std::chrono::steady_clock::time_point t0;
   };
@@ -111,7 +121,7 @@ template<typename T>
 int host_alloc(tile_stream_gpu* s, T** p, size_t count)
 {
   void* q = nullptr;
-  BCHECK(hipHostMalloc(&q, count * sizeof(T) + 256, hipHostMallocDefault));
+  BCHECK(host_alloc_near_gpu(&q, count * sizeof(T) + 256, s->device));
   s->host_allocations.push_back(q);
   *p = (T*)q;
   return 0;
@@ -150,12 +160,13 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
   if (codec.id == CODEC_LZ4_NON_STANDARD)
     BCHECK(launch_lz4(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                       s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
-                      s->d_seg_sizes + (uint64_t)slot * T * s->nseg, s->lz4_hash_log, st));
+                      s->d_seg_sizes + (uint64_t)slot * T * s->nseg, s->lz4_hash_log, s->lz4_seg,
+                      s->d_lz4_scratch, st));
   else if (codec.id == CODEC_ZSTD)
     BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                        s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
                        s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->zblock,
-                       s->d_zscratch, s->zworkers, s->d_queue, st));
+                       s->d_zscratch, s->zworkers, s->d_queue, s->d_ztables, st));
   return 0;
 }
 
@@ -174,8 +185,12 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
                                  P.d_sizes[lv], P.d_offsets[lv], st));
     BCHECK(hipMemcpyAsync(H.h_offsets[lv], P.d_offsets[lv], (C + 1) * 8, hipMemcpyDeviceToHost, st));
     BCHECK(hipMemcpyAsync(H.h_sizes[lv], P.d_sizes[lv], C * 8, hipMemcpyDeviceToHost, st));
-    BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
-                                   H.h_agg + H.level_base[lv], st, s->host_gather_wg));
+    if (b->dma)
+      BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
+                                     H.d_stage + H.level_base[lv], st, 0));
+    else
+      BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
+                                     H.h_agg + H.level_base[lv], st, s->host_gather_wg));
   }
   return 0;
 }
@@ -204,13 +219,74 @@ int capture(hipStream_t st, hipGraphExec_t* exec, F fn)
   return 0;
 }
 
-// Hands the oldest in-flight epoch to the sink once its gather is done.
+// DMA mode: queue the copies of a gathered epoch (its offsets are in pinned
+// memory once the aggregate graph has finished).
+int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t)
+{
+  auto& H = b->host[t.host_slot];
+  for (int lv = 0; lv < s->plan.nlod; ++lv) {
+    if (!(t.mask & (1u << lv)))
+      continue;
+    const uint64_t C = s->plan.levels[lv].agg.covering_count;
+    const uint64_t bytes = H.h_offsets[lv][C];
+    if (bytes)
+      BCHECK(hipMemcpyAsync(H.h_agg + H.level_base[lv], H.d_stage + H.level_base[lv], bytes,
+                            hipMemcpyDeviceToHost, b->st_copy));
+  }
+  t.copied = take_event(b);
+  if (!t.copied) {
+    set_error("batched: event creation failed");
+    return 1;
+  }
+  BCHECK(hipEventRecord(t.copied, b->st_copy));
+  return 0;
+}
+
+// DMA mode: queue copies for every in-flight epoch whose gather has finished
+// (in order, without blocking), so the copy engine always has work.
+int issue_ready_copies(tile_stream_gpu* s, BatchedState* b)
+{
+  for (auto& t : b->inflight) {
+    if (t.copied)
+      continue;
+    const hipError_t q = hipEventQuery(t.done);
+    if (q == hipErrorNotReady)
+      return 0;
+    BCHECK(q);
+    if (issue_copies(s, b, t))
+      return 1;
+  }
+  return 0;
+}
+
+// Hands the oldest in-flight epoch to the sink once its bytes are in host
+// memory. Returns 2 when `block` is false and it is not ready yet.
 int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
 {
   if (b->inflight.empty())
     return 0;
   auto& t = b->inflight.front();
-  if (block) {
+  if (b->dma) {
+    if (issue_ready_copies(s, b))
+      return 1;
+    if (!t.copied) {
+      if (!block)
+        return 2;
+      BCHECK(hipEventSynchronize(t.done));
+      if (issue_ready_copies(s, b))
+        return 1;
+    }
+    if (block) {
+      BCHECK(hipEventSynchronize(t.copied));
+    } else {
+      const hipError_t q = hipEventQuery(t.copied);
+      if (q == hipErrorNotReady)
+        return 2;
+      BCHECK(q);
+    }
+    b->event_pool.push_back(t.copied);
+    t.copied = nullptr;
+  } else if (block) {
     BCHECK(hipEventSynchronize(t.done));
   } else {
     const hipError_t q = hipEventQuery(t.done);
@@ -261,6 +337,8 @@ int batched_init(tile_stream_gpu* s)
   const Plan& p = s->plan;
   const uint32_t K = p.epochs_per_batch;
   BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
+  BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
+  b->dma = s->opts.batched_d2h != 1;
   b->R = std::min<uint32_t>(K, 4);
   // worst-case bytes of one epoch per level
   std::vector<uint64_t> level_bytes(p.nlod);
@@ -273,6 +351,8 @@ int batched_init(tile_stream_gpu* s)
   b->host.resize(b->R);
   for (auto& H : b->host) {
     if (host_alloc(s, &H.h_agg, total))
+      return 1;
+    if (b->dma && dev_alloc(s, &H.d_stage, total))
       return 1;
     H.level_base.assign(p.nlod, 0);
     H.h_offsets.assign(p.nlod, nullptr);
@@ -335,8 +415,12 @@ void batched_free(tile_stream_gpu* s)
   }
   for (auto& kv : b->agg_exec)
     (void)hipGraphExecDestroy(kv.second);
+  if (b->st_copy)
+    (void)hipStreamSynchronize(b->st_copy);
   for (auto& t : b->inflight) {
     (void)hipEventDestroy(t.done);
+    if (t.copied)
+      (void)hipEventDestroy(t.copied);
     if (t.c0)
       (void)hipEventDestroy(t.c0);
     if (t.c1)
@@ -348,6 +432,8 @@ void batched_free(tile_stream_gpu* s)
     (void)hipEventDestroy(e);
   if (b->st_agg)
     (void)hipStreamDestroy(b->st_agg);
+  if (b->st_copy)
+    (void)hipStreamDestroy(b->st_copy);
   delete b;
   s->batched = nullptr;
 }
@@ -401,7 +487,8 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     return 1;
   }
   BCHECK(hipEventRecord(done, b->st_agg));
-  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, c0, c1, std::chrono::steady_clock::now() });
+  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
+                          std::chrono::steady_clock::now() });
   // opportunistic in-order delivery of whatever has finished
   for (;;) {
     const int r = deliver_one(s, b, false);
@@ -435,6 +522,7 @@ int batched_drain(tile_stream_gpu* s)
         s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   BCHECK(hipStreamSynchronize(s->st_comp));
   BCHECK(hipStreamSynchronize(b->st_agg));
+  BCHECK(hipStreamSynchronize(b->st_copy));
   scatter_collect_ready(s);
   return 0;
 }

@@ -59,7 +59,7 @@ int halloc(tile_stream_gpu* s, void** p, size_t bytes)
   *p = nullptr;
   if (bytes == 0)
     return 0;
-  HIPCHECK(hipHostMalloc(p, bytes + kSlack, hipHostMallocDefault));
+  HIPCHECK(host_alloc_near_gpu(p, bytes + kSlack, s->device));
   s->host_allocations.push_back(*p);
   return 0;
 }
@@ -141,11 +141,12 @@ int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
   HIPCHECK(hipEventRecord(e0, s->st_comp));
   if (codec.id == CODEC_LZ4_NON_STANDARD) {
     HIPCHECK(launch_lz4(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
-                        s->chunk_slot, s->d_seg_sizes, s->lz4_hash_log, s->st_comp));
+                        s->chunk_slot, s->d_seg_sizes, s->lz4_hash_log, s->lz4_seg, s->d_lz4_scratch,
+                        s->st_comp));
   } else {
     HIPCHECK(launch_zstd(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
                          s->chunk_slot, s->d_seg_sizes, codec.level, s->zblock, s->d_zscratch, s->zworkers,
-                         s->d_queue, s->st_comp));
+                         s->d_queue, s->d_ztables, s->st_comp));
   }
   HIPCHECK(hipEventRecord(e1, s->st_comp));
   HIPCHECK(hipEventSynchronize(e1));
@@ -698,6 +699,11 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     s->chunk_slot = align_up(lz4_max_output_size(s->chunk_bytes), 256);
     s->seg_stride = s->chunk_slot;
     s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : 12;
+    const uint64_t per_launch = chunks_per_launch(s);
+    s->lz4_seg = tune_lz4_segment(s->chunk_bytes, per_launch, s->lz4_hash_log);
+    if (s->lz4_seg && dalloc(s, (void**)&s->d_lz4_scratch,
+                             lz4_scratch_bytes(s->chunk_bytes, per_launch, s->lz4_seg)))
+      return 1;
   } else if (config->codec.id == CODEC_ZSTD) {
     HIPCHECK(zstd_prepare());
     if (s->opts.zstd_block_bytes && !zstd_block_ok(s->opts.zstd_block_bytes)) {
@@ -706,6 +712,10 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     }
     s->zblock = s->opts.zstd_block_bytes ? s->opts.zstd_block_bytes
                                          : tune_zstd_block(s->chunk_bytes, chunks_per_launch(s));
+    const uint32_t ct = s->opts.zstd_chunk_tables;
+    const bool chunk_tables = ct == 1 || (ct == 0 && tune_zstd_chunk_tables(s->zblock));
+    if (chunk_tables && dalloc(s, (void**)&s->d_ztables, zstd_table_bytes(chunks_per_launch(s))))
+      return 1;
     s->nseg = zstd_segments(s->chunk_bytes, s->zblock);
     s->seg_stride = zstd_segment_stride(s->zblock);
     s->chunk_slot = zstd_max_output_size(s->chunk_bytes, s->zblock);
@@ -958,9 +968,21 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
   info->lod_bytes = lod;
   if (config->codec.id == CODEC_ZSTD) {
     const uint64_t blocks = nchunks * zstd_segments(cb, kZstdBlockMin);
-    info->codec_bytes = zstd_scratch_bytes((uint32_t)std::min<uint64_t>(blocks, 2048));
+    info->codec_bytes = zstd_scratch_bytes((uint32_t)std::min<uint64_t>(blocks, compressor_slots())) +
+                        zstd_table_bytes(nchunks);
   }
   info->codec_bytes += nchunks * (config->codec.id == CODEC_ZSTD ? zstd_segments(cb, kZstdBlockMin) : 1) * 4;
+  if (config->codec.id == CODEC_LZ4_NON_STANDARD) {
+    // segment scratch of the multi-wavefront LZ4 path (plain pipeline bound)
+    uint64_t epochs = K;
+    const uint64_t E = p.levels[0].layout.epoch_elements;
+    if (p.max_cursor_elements && E)
+      epochs = std::min<uint64_t>(epochs, (p.max_cursor_elements + E - 1) / E);
+    const uint64_t per_launch = std::max<uint64_t>(epochs, 1) * p.total_chunks;
+    const uint32_t seg = tune_lz4_segment(cb, per_launch, 12);
+    if (seg)
+      info->codec_bytes += lz4_scratch_bytes(cb, per_launch, seg);
+  }
   uint64_t shard = 0;
   for (int lv = 0; lv < p.nlod; ++lv)
     shard += p.levels[lv].shard_inner_count * (sizeof(OpenShard) + p.levels[lv].cps_total * 16);

@@ -55,9 +55,12 @@ struct tile_stream_gpu
   void* d_zscratch = nullptr;
   uint32_t zworkers = 0;
   uint32_t zblock = chucky::kZstdBlockMax;
+  uint8_t* d_ztables = nullptr; // per-chunk Huffman tables (chunk-table mode), else null
   uint32_t host_gather_wg = 64;
   uint32_t* d_queue = nullptr;
   int lz4_hash_log = 12;
+  uint32_t lz4_seg = 0; // segment bytes of the multi-wavefront LZ4 path (0 = one per chunk)
+  uint8_t* d_lz4_scratch = nullptr;
 
   // aggregate
   uint32_t* d_gather = nullptr;
@@ -128,8 +131,12 @@ void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes);
 std::recursive_mutex& graph_mutex();
 void scatter_collect_ready(tile_stream_gpu* s);
 uint32_t compressor_slots();
+// hipHostMalloc on the NUMA node of `device` (default policy if unknown).
+hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
 uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch);
 uint32_t tune_host_gather_workgroups();
+bool tune_zstd_chunk_tables(uint32_t block);
+uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log);
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
 int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);

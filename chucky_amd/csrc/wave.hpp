@@ -60,6 +60,16 @@ __device__ __forceinline__ void wave_copy(uint8_t* dst, const uint8_t* src, uint
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
+// Orders the LDS accesses of one wavefront across lanes (a wavefront issues
+// its LDS operations in order), for single-wavefront algorithms that may run
+// inside a larger workgroup.
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template<typename T>
 __device__ __forceinline__ T wave_bcast(T v, int src_lane)
 {

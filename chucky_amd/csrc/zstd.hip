@@ -41,7 +41,11 @@ namespace {
 using zcore::Seq;
 
 constexpr uint32_t kBlock = zcore::kBlockMax; // largest block (scratch is sized for it)
-constexpr uint64_t kSegSlack = 64; // segment = block + slack >= frame header + block header + raw block
+// segment = block + slack >= frame header + (chunk-table carrier block) +
+// block header + raw block
+constexpr uint64_t kSegSlack = 320;
+// bytes of a chunk's first block that carry its Huffman table
+constexpr uint32_t kCarrier = 64;
 constexpr int kHashLog = 12;
 constexpr uint32_t kMinMatch = 8;
 constexpr uint32_t kMaxSeq = 8192;
@@ -75,6 +79,15 @@ struct Lds
   uint32_t mode, tree_size, lit_size, lit_hsize, seq_size;
 };
 
+// Huffman table shared by every block of a chunk (chunk-table mode).
+struct ChunkTable
+{
+  uint32_t cw[256]; // code value | length << 16
+  uint8_t tree[192];
+  uint32_t tree_size; // 0 = no table: blocks use raw literals
+  uint32_t pad[3];
+};
+
 __device__ __forceinline__ uint32_t key_hash(uint64_t key)
 {
   return (uint32_t)((key * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
@@ -97,7 +110,7 @@ __device__ void wave_sort256(uint32_t* keys)
           keys[l] = a;
         }
       }
-      __syncthreads();
+      wave_sync();
     }
 }
 
@@ -135,6 +148,39 @@ __device__ __forceinline__ void count_bytes(const uint8_t* src, uint32_t n, uint
     if (run[k])
       atomicAdd(&hist[sym[k]], run[k]);
   for (uint32_t i = body + lane; i < n; i += kWave)
+    atomicAdd(&hist[src[i]], 1u);
+}
+
+// count_bytes for a whole workgroup (blockDim.x threads).
+__device__ __forceinline__ void count_bytes_block(const uint8_t* src, uint64_t n, uint32_t* hist)
+{
+  uint32_t sym[4] = { 0, 0, 0, 0 }, run[4] = { 0, 0, 0, 0 };
+  const uint64_t body = n & ~(uint64_t)(16 * 256 - 1);
+  for (uint64_t i = 16ull * threadIdx.x; i < body; i += 16ull * blockDim.x) {
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      w[j] = ld32u(src + i + 4 * j);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t b = (w[j] >> (8 * k)) & 0xFF;
+        if (b == sym[k]) {
+          ++run[k];
+        } else {
+          if (run[k])
+            atomicAdd(&hist[sym[k]], run[k]);
+          sym[k] = b;
+          run[k] = 1;
+        }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (run[k])
+      atomicAdd(&hist[sym[k]], run[k]);
+  for (uint64_t i = body + threadIdx.x; i < n; i += blockDim.x)
     atomicAdd(&hist[src[i]], 1u);
 }
 
@@ -244,45 +290,111 @@ __device__ void huf_build_wave(Lds& L, uint32_t n)
 {
   const uint32_t lane = (uint32_t)lane_id();
   constexpr uint32_t Lmax = zcore::kHufMaxBits;
-  Dist512 W, P;
+  // Two-queue Huffman merge. Leaf weights (ascending) and internal-node
+  // weights (created in ascending order) live in registers, 64 per register,
+  // lane j of register k holding entry 64k+j; the merge keeps both queue
+  // heads in scalars and refreshes only the popped head with one lane read.
+  // Parent links go to LDS (fire-and-forget stores by lane 0).
+  uint32_t lw[4], nw[4] = { 0, 0, 0, 0 };
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 4; ++k) {
     const uint32_t idx = (uint32_t)k * kWave + lane;
-    W.r[k] = (k < 4 && idx < n) ? L.hist[L.u.huf.sym[idx]] : 0u;
-    P.r[k] = 0;
+    lw[k] = idx < n ? L.hist[L.u.huf.sym[idx]] : 0u;
   }
-  uint32_t leaf = 0, node = n, next = n;
-  for (uint32_t k = 0; k + 1 < n; ++k) {
-    uint32_t pick0 = 0, pick1 = 0, wsum = 0;
+  auto pick4 = [](const uint32_t (&r)[4], uint32_t i) -> uint32_t {
+    uint32_t v;
+    switch (i >> 6) {
+      case 0: v = r[0]; break;
+      case 1: v = r[1]; break;
+      case 2: v = r[2]; break;
+      default: v = r[3]; break;
+    }
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(i & 63));
+  };
+  int16_t* parent = L.u.huf.parent;
+  uint32_t leaf = 0, node = 0, made = 0;
+  uint32_t hl = pick4(lw, 0), hn = 0xFFFFFFFFu;
+  for (uint32_t it = 0; it + 1 < n; ++it) {
+    uint32_t pk0 = 0, pk1 = 0, ws = 0;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const bool has_leaf = leaf < n, has_node = node < next;
-      const uint32_t wl = has_leaf ? dist_get(W, leaf) : 0xFFFFFFFFu;
-      const uint32_t wn = has_node ? dist_get(W, node) : 0xFFFFFFFFu;
-      uint32_t pick;
-      if (has_leaf && (!has_node || wl <= wn)) {
-        pick = leaf++;
-        wsum += wl;
+      uint32_t pk;
+      if (leaf < n && hl <= hn) { // ties go to the leaf
+        pk = leaf;
+        ws += hl;
+        ++leaf;
+        hl = leaf < n ? pick4(lw, leaf) : 0xFFFFFFFFu;
       } else {
-        pick = node++;
-        wsum += wn;
+        pk = n + node;
+        ws += hn;
+        ++node;
+        hn = node < made ? pick4(nw, node) : 0xFFFFFFFFu;
       }
       if (c == 0)
-        pick0 = pick;
+        pk0 = pk;
       else
-        pick1 = pick;
+        pk1 = pk;
     }
-    dist_set(W, next, wsum);
-    dist_set(P, pick0, next);
-    dist_set(P, pick1, next);
-    ++next;
+    const bool mine = lane == (made & 63);
+    switch (made >> 6) {
+      case 0: nw[0] = mine ? ws : nw[0]; break;
+      case 1: nw[1] = mine ? ws : nw[1]; break;
+      case 2: nw[2] = mine ? ws : nw[2]; break;
+      default: nw[3] = mine ? ws : nw[3]; break;
+    }
+    if (lane == 0) {
+      parent[pk0] = (int16_t)(n + made);
+      parent[pk1] = (int16_t)(n + made);
+    }
+    if (node == made)
+      hn = ws; // the node queue was empty: the new node is its head
+    ++made;
   }
-  Dist512 D;
+  const uint32_t nodes = n + made; // root = nodes - 1
+  if (lane == 0)
+    parent[nodes - 1] = (int16_t)(nodes - 1);
+  wave_sync();
+  // depths by pointer jumping: every node adds its parent's distance to the
+  // root and jumps to its grandparent; 9 rounds cover depths < 512
+  uint8_t* dep = L.u.huf.depth;
+  uint32_t pj[8], dj[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    D.r[k] = 0;
-  for (int i = (int)next - 2; i >= 0; --i)
-    dist_set(D, (uint32_t)i, dist_get(D, dist_get(P, (uint32_t)i)) + 1);
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t i = (uint32_t)k * kWave + lane;
+    pj[k] = i < nodes ? (uint32_t)parent[i] : i;
+    dj[k] = i < nodes && i != nodes - 1 ? 1u : 0u;
+  }
+  for (int round = 0; round < 9; ++round) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = (uint32_t)k * kWave + lane;
+      if (i < nodes) {
+        dep[i] = (uint8_t)dj[k];
+        parent[i] = (int16_t)pj[k];
+      }
+    }
+    wave_sync();
+    uint32_t dp[8], pp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = (uint32_t)k * kWave + lane;
+      dp[k] = i < nodes ? dep[pj[k]] : 0u;
+      pp[k] = i < nodes ? (uint32_t)parent[pj[k]] : pj[k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      dj[k] += dp[k];
+      pj[k] = pp[k];
+    }
+  }
+  struct
+  {
+    uint32_t r[4];
+  } D;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    D.r[k] = dj[k];
 
   // leaf depths of this lane (leaves k*64 + lane, k < 4)
   uint32_t len[4];
@@ -365,7 +477,7 @@ __device__ void huf_build_wave(Lds& L, uint32_t n)
   // scatter lengths to symbols
   for (uint32_t s = lane; s < 256; s += kWave)
     L.code.len[s] = 0;
-  __syncthreads();
+  wave_sync();
   uint32_t maxsym = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -377,7 +489,7 @@ __device__ void huf_build_wave(Lds& L, uint32_t n)
     }
   }
   maxsym = wave_max(maxsym);
-  __syncthreads();
+  wave_sync();
   // canonical values: per length, consecutive in symbol order, starting
   // values from the longest length down
   uint32_t sl[4];
@@ -405,13 +517,13 @@ __device__ void huf_build_wave(Lds& L, uint32_t n)
     L.code.max_bits = (int)maxd;
     L.code.max_sym = (int)maxsym;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // Compresses one block; returns bytes written at out (header included).
 __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
                                    uint8_t* lits, Seq* seqs, uint8_t* seqout,
-                                   const zcore::Predefined& pre)
+                                   const zcore::Predefined& pre, const ChunkTable* tab)
 {
   const int lane = lane_id();
 
@@ -511,7 +623,30 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       L.lit_size = raw_h + 1;
     }
   }
-  if (nlits >= 256 && used > 1) {
+  if (tab) {
+    // chunk-table mode: treeless literals with the chunk's code when that is
+    // smaller than raw (the carrier block at the start of the frame holds the
+    // tree description)
+    if (tab->tree_size && nlits >= 256 && used > 1) {
+      uint32_t bits_local = 0;
+      for (int s = lane; s < 256; s += kWave) {
+        const uint32_t c = tab->cw[s];
+        L.code.val[s] = (uint16_t)c;
+        L.code.len[s] = (uint8_t)(c >> 16);
+        bits_local += L.hist[s] * (c >> 16);
+      }
+      const uint32_t bits = wave_sum(bits_local);
+      if (lane == 0) {
+        const uint64_t comp_bound = 6 + (bits + 4 * 8 + 7) / 8;
+        const uint32_t hs = (uint32_t)zcore::huf_lits_header_size(nlits, nlits);
+        if (comp_bound < nlits && hs + comp_bound < L.lit_size) {
+          L.mode = 3;
+          L.lit_hsize = hs;
+          L.lit_size = hs + (uint32_t)comp_bound;
+        }
+      }
+    }
+  } else if (nlits >= 256 && used > 1) {
     // used symbols in ascending count order: bitonic sort of (count, symbol)
     uint32_t* keys = L.u.huf.w;
     for (int s = lane; s < 256; s += kWave)
@@ -566,7 +701,8 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       o[h] = lits[0];
     }
   } else {
-    const uint32_t hs = L.lit_hsize, ts = L.tree_size;
+    // mode 2: tree description + streams; mode 3: streams only (treeless)
+    const uint32_t hs = L.lit_hsize, ts = mode == 2 ? L.tree_size : 0u;
     if (lane == 0)
       for (uint32_t i = 0; i < ts; ++i)
         o[hs + i] = L.tree[i];
@@ -583,7 +719,7 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
     }
     const uint32_t comp = sp - hs;
     if (lane == 0) {
-      zcore::write_huf_lits_header(o, hs, nlits, comp);
+      zcore::write_huf_lits_header(o, hs, nlits, comp, mode);
       uint8_t* jt = o + hs + ts;
       for (int k = 0; k < 3; ++k) {
         jt[2 * k] = (uint8_t)sizes[k];
@@ -599,12 +735,99 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
   return 3 + body;
 }
 
+// First block of a chunk-table frame: the chunk's first `k` bytes as Huffman
+// literals in one stream, preceded by the tree description, and no sequences.
+// Every later treeless block of the frame decodes with this table. Lane 0.
+__device__ uint32_t write_carrier(uint8_t* out, const uint8_t* in, uint32_t k, const ChunkTable& t)
+{
+  uint8_t* lit = out + 3;
+  uint8_t* p = lit + 3;
+  for (uint32_t i = 0; i < t.tree_size; ++i)
+    p[i] = t.tree[i];
+  p += t.tree_size;
+  zcore::BitW bw;
+  zcore::bw_init(bw, p);
+  for (long i = (long)k - 1; i >= 0; --i) {
+    const uint32_t c = t.cw[in[i]];
+    zcore::bw_add(bw, c & 0xFFFF, c >> 16);
+  }
+  const uint32_t comp = t.tree_size + (uint32_t)zcore::bw_close(bw);
+  // compressed literals, size format 0: one stream, 10-bit sizes
+  const uint32_t v = 2u | (k << 4) | (comp << 14);
+  lit[0] = (uint8_t)v;
+  lit[1] = (uint8_t)(v >> 8);
+  lit[2] = (uint8_t)(v >> 16);
+  lit[3 + comp] = 0; // no sequences
+  const uint32_t body = 3 + comp + 1;
+  zcore::write_block_header(out, false, 2, body);
+  return 3 + body;
+}
+
+// One workgroup per chunk: byte histogram of the whole chunk, then the first
+// wavefront builds its Huffman code and tree description. The table is used
+// when its estimated stream size beats raw literals over the chunk.
+__global__ __launch_bounds__(256) void zstd_table_k(const uint8_t* __restrict__ src,
+                                                    uint64_t src_stride, uint32_t chunk_bytes,
+                                                    uint32_t nseg, uint64_t nchunks,
+                                                    ChunkTable* __restrict__ tables)
+{
+  __shared__ Lds L;
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= nchunks)
+    return;
+  for (uint32_t s = threadIdx.x; s < 256; s += blockDim.x)
+    L.hist[s] = 0;
+  __syncthreads();
+  count_bytes_block(src + chunk * src_stride, chunk_bytes, L.hist);
+  __syncthreads();
+  if (threadIdx.x >= kWave)
+    return;
+  const int lane = lane_id();
+  ChunkTable& t = tables[chunk];
+  uint32_t used_local = 0;
+  for (int s = lane; s < 256; s += kWave)
+    used_local += L.hist[s] != 0;
+  const uint32_t used = wave_sum(used_local);
+  uint32_t tree_size = 0;
+  if (used > 1 && chunk_bytes >= 4 * kCarrier) {
+    uint32_t* keys = L.u.huf.w;
+    for (int s = lane; s < 256; s += kWave)
+      keys[s] = (L.hist[s] << 8) | (uint32_t)s;
+    wave_sync();
+    wave_sort256(keys);
+    for (uint32_t i = lane; i < used; i += kWave)
+      L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
+    wave_sync();
+    huf_build_wave(L, used);
+    if (lane == 0)
+      L.tree_size = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
+    wave_sync();
+    uint64_t bits_local = 0;
+    for (int s = lane; s < 256; s += kWave)
+      bits_local += (uint64_t)L.hist[s] * L.code.len[s];
+    uint32_t bits_hi = wave_sum((uint32_t)(bits_local >> 16)), bits_lo = wave_sum((uint32_t)(bits_local & 0xFFFF));
+    const uint64_t bytes = (((uint64_t)bits_hi << 16) + bits_lo + 7) / 8;
+    tree_size = L.tree_size;
+    // worth it when the coded literals beat raw by more than the per-block
+    // stream overhead
+    if (!tree_size || bytes + tree_size + 16ull * nseg >= (uint64_t)chunk_bytes)
+      tree_size = 0;
+    for (int s = lane; s < 256; s += kWave)
+      t.cw[s] = (uint32_t)L.code.val[s] | ((uint32_t)L.code.len[s] << 16);
+    for (uint32_t i = lane; i < tree_size; i += kWave)
+      t.tree[i] = L.tree[i];
+  }
+  if (lane == 0)
+    t.tree_size = tree_size;
+}
+
 __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
                                              uint32_t chunk_bytes, uint32_t block, uint32_t nseg,
                                              uint64_t total_blocks, uint8_t* __restrict__ dst,
                                              uint64_t chunk_dst_stride,
                                              uint32_t* __restrict__ seg_sizes, uint8_t* scratch,
-                                             uint32_t* queue, const zcore::Predefined* __restrict__ pre)
+                                             uint32_t* queue, const zcore::Predefined* __restrict__ pre,
+                                             const ChunkTable* __restrict__ tables)
 {
   __shared__ Lds L;
   const int lane = lane_id();
@@ -627,13 +850,22 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
     const uint8_t* in = src + chunk * src_stride + (uint64_t)b * block;
     const uint32_t len = (chunk_bytes - b * block) < block ? chunk_bytes - b * block : block;
     uint8_t* out = dst + chunk * chunk_dst_stride + (uint64_t)b * (block + kSegSlack);
-    uint32_t hdr = 0;
+    const ChunkTable* tab = tables ? tables + chunk : nullptr;
+    uint32_t hdr = 0, skip = 0;
     if (b == 0) {
       if (lane == 0)
         zcore::write_frame_header(out, chunk_bytes);
       hdr = (uint32_t)zcore::kFrameHeaderBytes;
+      if (tab && tab->tree_size) {
+        uint32_t c = 0;
+        if (lane == 0)
+          c = write_carrier(out + hdr, in, kCarrier, *tab);
+        hdr += wave_bcast(c, 0);
+        skip = kCarrier;
+      }
     }
-    const uint32_t sz = compress_block(L, in, len, out + hdr, b + 1 == nseg, lits, seqs, seqout, *pre);
+    const uint32_t sz = compress_block(L, in + skip, len - skip, out + hdr, b + 1 == nseg, lits, seqs,
+                                       seqout, *pre, tab);
     if (lane == 0)
       seg_sizes[chunk * nseg + b] = hdr + sz;
     __syncthreads();
@@ -705,10 +937,12 @@ size_t zstd_max_output_size(size_t chunk_bytes, uint32_t block)
 }
 size_t zstd_scratch_bytes(uint32_t workers) { return (size_t)workers * kWorkerScratch; }
 
+size_t zstd_table_bytes(uint64_t n) { return (size_t)n * sizeof(ChunkTable); }
+
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
                        uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
-                       hipStream_t stream)
+                       void* tables, hipStream_t stream)
 {
   (void)level;
   if (n == 0)
@@ -720,17 +954,25 @@ hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_b
     return hipErrorNotInitialized; // zstd_prepare() was not called on this device
   const uint32_t nseg = zstd_segments(chunk_bytes, block);
   const uint64_t total = n * nseg;
+  ChunkTable* tabs = (ChunkTable*)tables;
+  if (tabs) {
+    hipLaunchKernelGGL(zstd_table_k, dim3((uint32_t)n), dim3(256), 0, stream, src, src_stride,
+                       (uint32_t)chunk_bytes, nseg, n, tabs);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+  }
   if (queue_counter) {
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(1), 0, stream, queue_counter);
     hipLaunchKernelGGL(zstd_k, dim3(workers), dim3(64), 0, stream, src, src_stride,
                        (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, queue_counter, pre);
+                       (uint8_t*)scratch, queue_counter, pre, tabs);
   } else {
     if (total > workers)
       return hipErrorInvalidValue; // one scratch slice per block in this mode
     hipLaunchKernelGGL(zstd_k, dim3((uint32_t)total), dim3(64), 0, stream, src, src_stride,
                        (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, nullptr, pre);
+                       (uint8_t*)scratch, nullptr, pre, tabs);
   }
   return hipGetLastError();
 }

@@ -685,15 +685,18 @@ ZHD inline size_t huf_lits_header_size(uint32_t regen, uint32_t comp)
   const uint32_t m = regen > comp ? regen : comp;
   return m < 1024 ? 3 : (m < 16384 ? 4 : 5);
 }
-ZHD inline void write_huf_lits_header(uint8_t* out, size_t hsize, uint32_t regen, uint32_t comp)
+// type 2: compressed literals (tree description follows); type 3: treeless
+// (the previous Huffman table of the frame is reused). Four streams.
+ZHD inline void write_huf_lits_header(uint8_t* out, size_t hsize, uint32_t regen, uint32_t comp,
+                                      uint32_t type = 2)
 {
   uint64_t v;
   if (hsize == 3)
-    v = 2u | (1u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 14);
+    v = type | (1u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 14);
   else if (hsize == 4)
-    v = 2u | (2u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 18);
+    v = type | (2u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 18);
   else
-    v = 2u | (3u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 22);
+    v = type | (3u << 2) | ((uint64_t)regen << 4) | ((uint64_t)comp << 22);
   for (size_t i = 0; i < hsize; ++i)
     out[i] = (uint8_t)(v >> (8 * i));
 }

@@ -34,6 +34,9 @@ struct tile_stream_gpu_options
   uint32_t zstd_block_bytes; /* zstd block size (<= 131072); 0 = autotuned */
   uint32_t host_gather_workgroups; /* CU budget of the gather into pinned host memory
                                       (batched pipeline); 0 = autotuned */
+  uint32_t zstd_chunk_tables;      /* one Huffman table per chunk: 0 = autotuned, 1 = on, 2 = off */
+  uint32_t batched_d2h;            /* batched pipeline: 0/2 = gather on the device + DMA copy,
+                                      1 = gather kernel writing pinned host memory */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);

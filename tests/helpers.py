@@ -134,9 +134,9 @@ def decoded_chunks(shards, cps_total_of_level, codec, chunk_bytes):
 
 # ---- runners ------------------------------------------------------------------
 
-def run_gpu(cfg, data, pieces=1, pipeline=0, device_input=False):
+def run_gpu(cfg, data, pieces=1, pipeline=0, device_input=False, **options):
     sink = ca.MemSink()
-    s = ca.GpuStream(cfg, sink, pipeline=pipeline)
+    s = ca.GpuStream(cfg, sink, pipeline=pipeline, **options)
     flat = np.ascontiguousarray(data).reshape(-1)
     if device_input:
         dev = ca.DeviceBuffer.from_host(flat)

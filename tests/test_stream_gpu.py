@@ -51,10 +51,10 @@ def _where(got: bytes, want: bytes, cps_total: int, chunk_bytes: int, dt) -> str
     return "; ".join(msgs) or "data equal, bytes outside chunks differ"
 
 
-def _compare(cfg, data, pieces=1, device_input=False, pipeline=0):
+def _compare(cfg, data, pieces=1, device_input=False, pipeline=0, **options):
     g, _, chunk_bytes = _geom(cfg)
     gs, gmeta, gcur = run_gpu(cfg, data, pieces=pieces, device_input=device_input,
-                              pipeline=pipeline)
+                              pipeline=pipeline, **options)
     rs, rmeta, rcur = run_ref(cfg, data, pieces=pieces)
     assert gcur == rcur
     assert set(gs) == set(rs), (sorted(gs), sorted(rs))
@@ -199,6 +199,46 @@ def test_drain_batch_lut_reuse(rows, K, codec, pipeline):
     cfg = ca.Config([Dim(rows, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
                      Dim(20, 4, 0, downsample=True)], dtype="f32", codec=codec, epochs_per_batch=K)
     _compare(cfg, random_typed(rows * 24 * 20, "f32", seed=33), pieces=3, pipeline=pipeline)
+
+
+LZ4_SEGMENT_DATA = {
+    "xor": lambda: xor_volume((64, 64, 64)),
+    "thirds": lambda: thirds_volume((64, 64, 64)),
+    "random_u16": lambda: random_typed(64 * 64 * 64, "u16", seed=5),
+    "sensor_like": lambda: np.clip(np.random.default_rng(3).poisson(100, 64 * 64 * 64), 0, 4095)
+    .astype(np.uint16),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LZ4_SEGMENT_DATA))
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_lz4_multi_wavefront_chunks(name, pipeline):
+    # 128 KiB chunks, few per launch: each chunk is parsed in 16 KiB segments
+    # by separate wavefronts and stitched into one LZ4 block
+    cfg = ca.Config([Dim(64, 32, 2), Dim(64, 32, 2), Dim(64, 64, 1)], dtype="u16", codec="lz4")
+    _compare(cfg, LZ4_SEGMENT_DATA[name](), pieces=2, pipeline=pipeline)
+
+
+ZSTD_MODE_DATA = {
+    "xor": lambda: xor_volume((32, 128, 128)),
+    "sensor_like": lambda: np.clip(np.random.default_rng(4).poisson(100, 32 * 128 * 128), 0, 4095)
+    .astype(np.uint16),
+    "thirds": lambda: thirds_volume((32, 128, 192))[:, :, :128].copy(),
+    "random_u8": lambda: random_typed(32 * 128 * 128 * 2, "u8", seed=9),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ZSTD_MODE_DATA))
+@pytest.mark.parametrize("block,tables", [(8192, 1), (8192, 2), (32768, 1), (131072, 1), (131072, 2)])
+def test_zstd_block_sizes_and_table_modes(name, block, tables):
+    # per-block Huffman tables (2) or one table per chunk carried by the first
+    # block and reused as treeless literals (1), at several block sizes
+    dt = "u8" if name == "random_u8" else "u16"
+    shape_x = 256 if dt == "u8" else 128
+    cfg = ca.Config([Dim(32, 16, 1), Dim(128, 64, 2), Dim(shape_x, 64 if dt == "u16" else 128, 0)],
+                    dtype=dt, codec="zstd")
+    data = ZSTD_MODE_DATA[name]()
+    _compare(cfg, data, pieces=2, zstd_block_bytes=block, zstd_chunk_tables=tables)
 
 
 def test_multiscale_dim0_zstd():
