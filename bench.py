@@ -276,6 +276,37 @@ def serving(args):
     }
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
+    scaled by a copy of known size), or None when none covers the pair."""
+    if workload != "sensor":
+        return None
+    path = os.path.join(ROOT, "profiles", "zstd_k_sensor512x4096x2304_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(nbytes):
+    """One identity-layout scatter of nbytes (a plain 16-byte-vector copy):
+    the known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    import chucky_amd as ca
+    n = nbytes // 4
+    src, dst = ca.DeviceBuffer(n * 4), ca.DeviceBuffer(n * 4)
+    shape, strides = (C.c_uint64 * 1)(n), (C.c_int64 * 1)(1)
+    t0 = time.perf_counter()
+    if ca.lib.chucky_gpu_transpose(dst.ptr, src.ptr, n, 4, 0, 1, shape, strides):
+        raise RuntimeError("copy failed: " + ca.last_error())
+    dt = time.perf_counter() - t0
+    return {"copy_bytes": n * 4, "kernel": "transpose_vec_k<4>", "seconds": round(dt, 6)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +322,9 @@ def main():
     ap.add_argument("--zstd-block", type=int, default=0, help="zstd block bytes (0 = autotuned)")
     ap.add_argument("--gather-wg", type=int, default=0,
                     help="workgroups of the gather into pinned host memory (0 = autotuned)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device copy of BYTES (the vector scatter kernel over an "
+                         "identity layout), the reference for counter-based HBM traffic")
     ap.add_argument("--batched-d2h", type=int, default=0,
                     help="batched pipeline D2H: 0 = device gather + DMA, 1 = gather kernel into host")
     args = ap.parse_args()
@@ -299,6 +333,9 @@ def main():
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        print(json.dumps(calibrate_copy(args.calibrate_copy)))
+        return
     if args.workload == "serving":
         print(json.dumps(serving(args)))
         return
@@ -352,7 +389,9 @@ def main():
         "pipelines": {k: summarize(v, args.steps)[1] for k, v in runs.items()},
         "roofline": {"bound": "hbm", "kernel": "zstd_k" if "zstd" in head["desc"] else "lz4_chunk_k",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(args.workload,
+                                                 "zstd_k" if "zstd" in head["desc"] else "lz4_chunk_k"),
                      "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
                      "note": "compressor input bytes per launch / event-timed launch duration"},
         "transpose": {"kernel": "transpose_vec_k", "achieved_GBps": round(sc_gbps, 1),
