@@ -53,6 +53,12 @@ def workload(name, frames):
         desc = (f"{frames} frames of 4096x2304 u16, chunks 64x256x256, shards 1x8x9 chunks, "
                 "1 scale, zstd level 1, discard sink")
         return cfg, (frames, 4096, 2304), desc
+    if name == "lod":
+        cfg = ca.Config([Dim(frames, 16, 1, name="t"), Dim(2048, 256, 8, name="y", downsample=True),
+                         Dim(2048, 256, 8, name="x", downsample=True)], dtype="u16", codec="none",
+                        reduce="mean", buffer_capacity_bytes=64 << 20)
+        return cfg, (frames, 2048, 2048), (f"{frames} frames of 2048x2048 u16, mean 2x2 LOD over y,x "
+                                          "(4 levels), chunks 16x256x256, no codec, discard sink")
     n = 256
     if name == "volume":
         cfg = ca.Config([Dim(n, 64, 4, name="z"), Dim(n, 64, 4, name="y"), Dim(n, 64, 4, name="x")],
@@ -82,6 +88,12 @@ def make_input(name, shape, seed=20260403):
         if ca.lib.chucky_synth_sensor_frames(buf.ptr, frames, H, W, seed, 100.0,
                                              blobs.ctypes.data_as(C.POINTER(C.c_float)), nb):
             raise RuntimeError("synthetic frame generation failed: " + ca.last_error())
+        return buf, None
+    if name == "lod":
+        n = int(np.prod(shape))
+        buf = ca.DeviceBuffer(n * 2)
+        if ca.lib.chucky_synth_fill(buf.ptr, n, ca.DTYPE_ID["u16"], seed, 4096):
+            raise RuntimeError("fill failed: " + ca.last_error())
         return buf, None
     from tests.helpers import xor_volume
     vol = xor_volume(shape)
@@ -123,16 +135,33 @@ def run(name, frames, steps, warmup, pipeline, host_input, options=None):
     dt = time.perf_counter() - t0
     m1 = snapshot(s)
     res = dict(seconds=dt, bytes=nbytes, out_bytes=(sink.bytes - b0) / max(steps, 1), desc=desc,
-               metrics={k: tuple(m1[k][i] - m0[k][i] for i in range(3)) for k in m1},
+               metrics={k: tuple(m1[k][i] - m0[k][i] for i in range(4)) for k in m1},
                latencies=lat, ms_per_step=dt * 1e3 / max(steps, 1), step_ms=step_ms)
     s.close()
     return res, data, cfg
 
 
+LOD_PASSES = {"lod_gather": "tiles_k (linear epoch -> level-0 Morton)",
+              "lod_reduce": "reduce_k (level l -> l+1, all levels)",
+              "lod_append_fold": "fold_k + emit_k (append-dim fold)",
+              "lod_morton_chunk": "tiles_k (Morton levels -> chunk pool, all levels)"}
+
+
 def snapshot(s):
     m = s.metrics()
-    return {n: (getattr(m, n).ms, getattr(m, n).count, getattr(m, n).input_bytes)
-            for n in ("scatter", "compress", "aggregate", "d2h", "sink")}
+    return {n: (getattr(m, n).ms, getattr(m, n).count, getattr(m, n).input_bytes,
+                getattr(m, n).output_bytes)
+            for n in ("scatter", "compress", "aggregate", "d2h", "sink", *LOD_PASSES)}
+
+
+def pass_rate(metric):
+    """(avg ms per launch, algorithmic bytes read+written per launch, GB/s)."""
+    ms, n, rd, wr = metric
+    if not n or ms <= 0:
+        return 0.0, 0.0, 0.0
+    avg = ms / n
+    per = (rd + wr) / n
+    return avg, per, per / (avg / 1e3) / 1e9
 
 
 def cpu_baseline(name, data, frames_total, sample_frames):
@@ -164,12 +193,12 @@ def cpu_baseline(name, data, frames_total, sample_frames):
 def summarize(res, steps):
     value = res["bytes"] * steps / res["seconds"] / 1e9
     m = res["metrics"]
-    comp_ms, comp_n, comp_in = m["compress"]
+    comp_ms, comp_n, comp_in, _ = m["compress"]
     lat = sorted(res["latencies"])
     pct = (lambda q: round(lat[min(len(lat) - 1, int(q * len(lat)))] * 1e3, 3)) if lat else (lambda q: None)
     stages = {k: {"ms": round(v[0] / steps, 3), "launches": v[1] // max(steps, 1),
                   "GBps": round(v[2] / v[0] / 1e6, 2) if v[0] > 0 else None}
-              for k, v in m.items() if v[1]}
+              for k, v in m.items() if v[1] and k not in LOD_PASSES}
     return value, dict(GBps=round(value, 3), ms_per_step=round(res["ms_per_step"], 3),
                        ratio=round(res["bytes"] / max(res["out_bytes"], 1), 4),
                        p50_epoch_latency_ms=pct(0.50), p99_epoch_latency_ms=pct(0.99),
@@ -280,9 +309,10 @@ def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
     scaled by a copy of known size), or None when none covers the pair."""
-    if workload != "sensor":
+    files = {"sensor": "zstd_k_sensor512x4096x2304_pmc.md", "lod": "tiles_k_lod64x2048x2048_pmc.md"}
+    if workload not in files:
         return None
-    path = os.path.join(ROOT, "profiles", "zstd_k_sensor512x4096x2304_pmc.md")
+    path = os.path.join(ROOT, "profiles", files[workload])
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
@@ -313,7 +343,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=512)
-    ap.add_argument("--workload", choices=["sensor", "volume", "multiscale", "serving"],
+    ap.add_argument("--workload", choices=["sensor", "volume", "multiscale", "lod", "serving"],
                     default="sensor")
     ap.add_argument("--pipeline", choices=["plain", "batched", "both"], default="both")
     ap.add_argument("--host-input", action="store_true")
@@ -353,13 +383,36 @@ def main():
     head = runs["batched" if "batched" in runs else order[0]]
     value, head_sum = summarize(head, args.steps)
     m = head["metrics"]
-    comp_ms, comp_n, comp_in = m["compress"]
+    comp_ms, comp_n, comp_in, _ = m["compress"]
     avg_ms = comp_ms / max(comp_n, 1)
     bytes_per_launch = comp_in / max(comp_n, 1)
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
-    sc_ms, sc_n, sc_in = m["scatter"]
+    sc_ms, sc_n, sc_in, _ = m["scatter"]
     sc_avg = sc_ms / max(sc_n, 1)
     sc_gbps = 2 * (sc_in / max(sc_n, 1)) / (sc_avg / 1e3) / 1e9 if sc_avg > 0 else 0.0
+    lod = {}
+    for k, label in LOD_PASSES.items():
+        avg, per, gbps = pass_rate(m[k])
+        if per:
+            lod[k] = {"kernels": label, "avg_launch_ms": round(avg, 4), "bytes_per_launch": per,
+                      "achieved_GBps": round(gbps, 1), "frac_of_hbm_peak": round(gbps / HBM_PEAK_GBPS, 4)}
+    codec_none = "no codec" in head["desc"]
+    if codec_none and "lod_gather" in lod:
+        # no compressor: the dominant kernel is the LOD gather (one launch per epoch)
+        g = lod["lod_gather"]
+        roofline = {"bound": "hbm", "kernel": "tiles_k (LOD gather)", "achieved": g["achieved_GBps"],
+                    "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": g["frac_of_hbm_peak"],
+                    "traffic": measured_traffic(args.workload, "tiles_k"),
+                    "avg_launch_ms": g["avg_launch_ms"], "bytes_per_launch": g["bytes_per_launch"],
+                    "note": "epoch bytes read + Morton bytes written per launch / event-timed launch duration"}
+    else:
+        kname = "zstd_k" if "zstd" in head["desc"] else "lz4_chunk_k"
+        roofline = {"bound": "hbm", "kernel": kname,
+                    "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": measured_traffic(args.workload, kname),
+                    "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
+                    "note": "compressor input bytes per launch / event-timed launch duration"}
 
     cpu = None
     if not args.no_cpu_baseline and data is not None:
@@ -387,18 +440,15 @@ def main():
         "p50_epoch_latency_ms": head_sum["p50_epoch_latency_ms"],
         "p99_epoch_latency_ms": head_sum["p99_epoch_latency_ms"],
         "pipelines": {k: summarize(v, args.steps)[1] for k, v in runs.items()},
-        "roofline": {"bound": "hbm", "kernel": "zstd_k" if "zstd" in head["desc"] else "lz4_chunk_k",
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_traffic(args.workload,
-                                                 "zstd_k" if "zstd" in head["desc"] else "lz4_chunk_k"),
-                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
-                     "note": "compressor input bytes per launch / event-timed launch duration"},
-        "transpose": {"kernel": "transpose_vec_k", "achieved_GBps": round(sc_gbps, 1),
-                      "avg_launch_ms": round(sc_avg, 4),
-                      "frac_of_hbm_peak": round(sc_gbps / HBM_PEAK_GBPS, 4)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if sc_n:
+        line["transpose"] = {"kernel": "transpose_vec_k", "achieved_GBps": round(sc_gbps, 1),
+                             "avg_launch_ms": round(sc_avg, 4),
+                             "frac_of_hbm_peak": round(sc_gbps / HBM_PEAK_GBPS, 4)}
+    if lod:
+        line["lod"] = lod
     if "plain" in runs and "batched" in runs:
         line["batched_over_plain"] = round(runs["plain"]["seconds"] / runs["batched"]["seconds"], 3)
     print(json.dumps(line))

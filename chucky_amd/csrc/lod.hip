@@ -12,11 +12,27 @@
 //   scatter  Morton level values -> chunk pool of the active levels
 // Each output element is owned by one lane and its window is reduced
 // sequentially in Morton order, so float sums are bit-identical to the CPU
-// path. The pass is HBM-bound integer/byte work: no LDS staging is needed for
-// the windowed reads, which are contiguous runs of 2^lod_ndim elements.
+// path. The pass is HBM-bound integer/byte work.
+//
+// gather and scatter move whole tiles: an aligned 2^s cube of the LOD dims
+// (1024 cells in 1-D and 2-D, 512 in 3-D) that lies inside the box (and, in the
+// chunk pool, inside one chunk) is a contiguous run of Morton ranks, and its
+// cells sit at origin + sum(c_k * stride_k) on the array side. A workgroup
+// stages one tile in LDS so that both the strided side (row by row) and the
+// Morton side (one run) are read and written in whole rows. Cells outside full
+// tiles go through a per-cell (rank, offset) list. The host builds both lists
+// and checks every tile against the rank LUT.
+//
+// reduce: a block of 256 outputs whose windows all have 2^lod_ndim cells (the
+// interior of the box) skips the window-end lookups and reduces fixed-length
+// runs; ragged blocks read the ends table.
 #include "kernels.hpp"
 #include "stream_gpu.hpp"
 
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace chucky {
@@ -33,49 +49,205 @@ template<> struct Acc<_Float16> { using type = float; };
 
 template<typename T> constexpr bool is_fp = std::is_floating_point<T>::value || std::is_same<T, _Float16>::value;
 
-template<typename T>
-__global__ __launch_bounds__(256) void gather_k(const T* __restrict__ linear, T* __restrict__ morton,
-                                                const uint32_t* __restrict__ lut,
-                                                const uint64_t* __restrict__ fixed, uint64_t n0,
-                                                uint64_t total)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct TileGeom
 {
-  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= total)
-    return;
-  const uint64_t b = g / n0, m = g - b * n0;
-  morton[g] = linear[fixed[b] + lut[m]];
+  uint32_t shift[3];  // position of coordinate k in the row-major tile index
+  int64_t stride[3];  // offset step of row-major position q (0 = slowest)
+};
+
+// Local Z index -> row-major index: coordinate k takes bits k, k+D, k+2D, ...
+template<int D, int S>
+__device__ __forceinline__ uint32_t z_to_row(uint32_t z, const TileGeom& g)
+{
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      c |= ((z >> (j * D + k)) & 1u) << j;
+    r |= c << g.shift[k];
+  }
+  return r;
 }
 
-template<typename T, int M>
-__device__ __forceinline__ T reduce_window(const T* v, uint64_t s, uint64_t e)
+template<int D, int S>
+__device__ __forceinline__ int64_t row_offset(uint32_t r, const TileGeom& g)
 {
-  const uint64_t len = e - s;
+  int64_t off = 0;
+#pragma unroll
+  for (int q = 0; q < D; ++q)
+    off += (int64_t)((r >> (S * (D - 1 - q))) & ((1u << S) - 1)) * g.stride[q];
+  return off;
+}
+
+// One tile per workgroup and batch: ToMorton gathers array -> Morton run,
+// otherwise scatters Morton run -> array. U is an unsigned integer of the
+// element size (the pass only moves bytes). Both sides move in 16-byte pieces
+// when aligned: the array side when VecArray (the host found every row start
+// aligned, and the base pointer is), the Morton side when its run start is.
+// A piece of V consecutive Z indices maps to rows z_to_row(z) + z_to_row(j),
+// j < V, since the bits of z (a multiple of V) and of j are disjoint. LDS
+// rows are padded by one piece, which spreads the Z-order accesses of a wave
+// over the banks; the padded index is linear over disjoint bits as well.
+template<typename U, int D, bool ToMorton, bool VecArray>
+__global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __restrict__ array,
+                                               const uint32_t* __restrict__ tiles,
+                                               const uint64_t* __restrict__ fixed, uint64_t n,
+                                               uint32_t nb, TileGeom g)
+{
+  constexpr int S = lod_tile_side_log(D, sizeof(U));
+  constexpr uint32_t cells = 1u << (S * D);
+  constexpr uint32_t V = 16 / sizeof(U);
+  constexpr uint32_t pieces = cells / V;
+  constexpr uint32_t kPieceIters = (pieces + 255) / 256;
+  constexpr uint32_t pad = D > 1 ? V : 0;
+  static_assert(cells % 256 == 0, "a tile spans whole workgroups");
+  __shared__ __align__(16) U buf[cells + (cells >> S) * pad];
+  auto at = [](uint32_t r) { return r + (r >> S) * pad; };
+  uint32_t jr[V];
+#pragma unroll
+  for (uint32_t j = 0; j < V; ++j)
+    jr[j] = at(z_to_row<D, S>(j, g));
+  const uint32_t base = tiles[2 * blockIdx.x], origin = tiles[2 * blockIdx.x + 1];
+  for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
+    U* m = morton + (uint64_t)b * n + base;
+    U* a = array + fixed[b] + origin;
+    const bool mvec = ((uintptr_t)m & 15) == 0;
+    if constexpr (ToMorton) {
+      if constexpr (VecArray) {
+#pragma unroll
+        for (uint32_t i = 0; i < kPieceIters; ++i) {
+          const uint32_t u = i * 256 + threadIdx.x;
+          if (pieces % 256 == 0 || u < pieces)
+            *(uint4*)&buf[at(u * V)] = *(const uint4*)(a + row_offset<D, S>(u * V, g));
+        }
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < cells / 256; ++i) {
+          const uint32_t r = i * 256 + threadIdx.x;
+          buf[at(r)] = a[row_offset<D, S>(r, g)];
+        }
+      }
+      __syncthreads();
+      if (mvec) {
+#pragma unroll
+        for (uint32_t i = 0; i < kPieceIters; ++i) {
+          const uint32_t u = i * 256 + threadIdx.x;
+          if (pieces % 256 != 0 && u >= pieces)
+            break;
+          const uint32_t zr = at(z_to_row<D, S>(u * V, g));
+          U e[V];
+#pragma unroll
+          for (uint32_t j = 0; j < V; ++j)
+            e[j] = buf[zr + jr[j]];
+          u32x4 w;
+          __builtin_memcpy(&w, e, 16);
+          // streamed out: nontemporal stores measured faster for this pass
+          __builtin_nontemporal_store(w, (u32x4*)(m + u * V));
+        }
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < cells / 256; ++i) {
+          const uint32_t z = i * 256 + threadIdx.x;
+          m[z] = buf[at(z_to_row<D, S>(z, g))];
+        }
+      }
+    } else {
+      if (mvec) {
+#pragma unroll
+        for (uint32_t i = 0; i < kPieceIters; ++i) {
+          const uint32_t u = i * 256 + threadIdx.x;
+          if (pieces % 256 != 0 && u >= pieces)
+            break;
+          const uint32_t zr = at(z_to_row<D, S>(u * V, g));
+          const uint4 w = *(const uint4*)(m + u * V);
+          U e[V];
+          __builtin_memcpy(e, &w, 16);
+#pragma unroll
+          for (uint32_t j = 0; j < V; ++j)
+            buf[zr + jr[j]] = e[j];
+        }
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < cells / 256; ++i) {
+          const uint32_t z = i * 256 + threadIdx.x;
+          buf[at(z_to_row<D, S>(z, g))] = m[z];
+        }
+      }
+      __syncthreads();
+      if constexpr (VecArray) {
+#pragma unroll
+        for (uint32_t i = 0; i < kPieceIters; ++i) {
+          const uint32_t u = i * 256 + threadIdx.x;
+          if (pieces % 256 == 0 || u < pieces)
+            __builtin_nontemporal_store(*(const u32x4*)&buf[at(u * V)], (u32x4*)(a + row_offset<D, S>(u * V, g)));
+        }
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < cells / 256; ++i) {
+          const uint32_t r = i * 256 + threadIdx.x;
+          a[row_offset<D, S>(r, g)] = buf[at(r)];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Cells outside full tiles: one (rank, offset) pair per work-item.
+template<typename T, bool ToMorton>
+__global__ __launch_bounds__(256) void edge_k(T* __restrict__ morton, T* __restrict__ array,
+                                              const uint32_t* __restrict__ edge, uint32_t ne,
+                                              const uint64_t* __restrict__ fixed, uint64_t n,
+                                              uint32_t nb)
+{
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ne)
+    return;
+  const uint32_t mi = edge[2 * i], off = edge[2 * i + 1];
+  for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
+    if constexpr (ToMorton)
+      morton[(uint64_t)b * n + mi] = array[fixed[b] + off];
+    else
+      array[fixed[b] + off] = morton[(uint64_t)b * n + mi];
+  }
+}
+
+// Reduce the run v[0, len). With len a compile-time constant (the regular
+// path) the loops unroll; the operations and their order are the same.
+template<typename T, int M>
+__device__ __forceinline__ T reduce_run(const T* v, uint32_t len)
+{
+  const uint32_t s = 0, e = len;
   if constexpr (M == lod_reduce_mean) {
     using A = typename Acc<T>::type;
     A sum = 0;
-    for (uint64_t j = s; j < e; ++j)
+    for (uint32_t j = s; j < e; ++j)
       sum += (A)v[j];
     return (T)(sum / (A)len);
   } else if constexpr (M == lod_reduce_min) {
     T best = v[s];
-    for (uint64_t j = s + 1; j < e; ++j)
+    for (uint32_t j = s + 1; j < e; ++j)
       if (v[j] < best)
         best = v[j];
     return best;
   } else if constexpr (M == lod_reduce_max) {
     T best = v[s];
-    for (uint64_t j = s + 1; j < e; ++j)
+    for (uint32_t j = s + 1; j < e; ++j)
       if (v[j] > best)
         best = v[j];
     return best;
   } else if constexpr (M == lod_reduce_median) {
     T buf[16];
-    const uint64_t n = len <= 16 ? len : 16;
-    for (uint64_t j = 0; j < n; ++j)
+    const uint32_t n = len <= 16 ? len : 16;
+    for (uint32_t j = 0; j < n; ++j)
       buf[j] = v[s + j];
-    for (uint64_t i = 1; i < n; ++i) {
+    for (uint32_t i = 1; i < n; ++i) {
       const T key = buf[i];
-      uint64_t k = i;
+      uint32_t k = i;
       while (k > 0 && buf[k - 1] > key) {
         buf[k] = buf[k - 1];
         --k;
@@ -85,7 +257,7 @@ __device__ __forceinline__ T reduce_window(const T* v, uint64_t s, uint64_t e)
     return buf[n / 2];
   } else if constexpr (M == lod_reduce_max_suppressed) {
     T t1 = v[s], t2 = v[s];
-    for (uint64_t j = s + 1; j < e; ++j) {
+    for (uint32_t j = s + 1; j < e; ++j) {
       const T x = v[j];
       if (x >= t1) {
         t2 = t1;
@@ -96,7 +268,7 @@ __device__ __forceinline__ T reduce_window(const T* v, uint64_t s, uint64_t e)
     return t2;
   } else {
     T b1 = v[s], b2 = v[s];
-    for (uint64_t j = s + 1; j < e; ++j) {
+    for (uint32_t j = s + 1; j < e; ++j) {
       const T x = v[j];
       if (x <= b1) {
         b2 = b1;
@@ -108,17 +280,115 @@ __device__ __forceinline__ T reduce_window(const T* v, uint64_t s, uint64_t e)
   }
 }
 
-template<typename T, int M>
+// reduce_run over a register window x[0, W) with compile-time indices, so it
+// stays in registers. The median sorts by compare-and-swap passes that make
+// exactly the moves of the insertion sort (each pass carries the new element
+// down while its left neighbour is greater; the prefix has no adjacent
+// inversion, so the pass makes no further swap).
+template<typename T, int M, int W>
+__device__ __forceinline__ T reduce_fixed(const T* x)
+{
+  if constexpr (M == lod_reduce_median) {
+    T buf[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      buf[j] = x[j];
+#pragma unroll
+    for (int i = 1; i < W; ++i)
+#pragma unroll
+      for (int k = i; k > 0; --k)
+        if (buf[k - 1] > buf[k]) {
+          const T t = buf[k - 1];
+          buf[k - 1] = buf[k];
+          buf[k] = t;
+        }
+    return buf[W / 2];
+  } else {
+    return reduce_run<T, M>(x, W);
+  }
+}
+
+template<int N>
+__device__ __forceinline__ void copy_bytes(void* __restrict__ dst, const void* __restrict__ src)
+{
+  if constexpr (N % 16 == 0) {
+#pragma unroll
+    for (int i = 0; i < N / 16; ++i)
+      ((uint4*)dst)[i] = ((const uint4*)src)[i];
+  } else if constexpr (N == 8) {
+    *(uint64_t*)dst = *(const uint64_t*)src;
+  } else if constexpr (N == 4) {
+    *(uint32_t*)dst = *(const uint32_t*)src;
+  } else if constexpr (N == 2) {
+    *(uint16_t*)dst = *(const uint16_t*)src;
+  } else {
+    *(uint8_t*)dst = *(const uint8_t*)src;
+  }
+}
+
+constexpr uint32_t outputs_per_lane(int W, size_t sz)
+{
+  return W == 0 || W * sz >= 32 ? 1u : (uint32_t)(32 / (W * sz));
+}
+
+// Level l -> l+1 for dst_n outputs per batch; W = 2^lod_ndim. A workgroup owns
+// 256*P consecutive outputs. When all their windows have W cells (the
+// interior) each lane reduces P consecutive windows read as one run of
+// W*P elements (32 bytes or more, in 16-byte pieces when aligned); otherwise
+// lanes take outputs i0 + p*256 + lane and read the window ends.
+template<typename T, int M, int W>
 __global__ __launch_bounds__(256) void reduce_k(T* __restrict__ values, uint64_t src_beg,
                                                 uint64_t dst_beg, const uint64_t* __restrict__ ends,
-                                                uint64_t src_n, uint64_t dst_n, uint64_t total)
+                                                uint32_t src_n, uint32_t dst_n, uint32_t nb)
 {
-  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= total)
+  constexpr uint32_t P = outputs_per_lane(W, sizeof(T));
+  const uint32_t i0 = blockIdx.x * (256 * P);
+  const uint32_t i1 = min(i0 + 256 * P, dst_n);
+  const uint64_t s0 = i0 ? ends[i0 - 1] : 0;
+  // windows hold 1..W cells, so W cells per window on average means all W
+  const bool regular = W && ends[i1 - 1] - s0 == (uint64_t)W * (i1 - i0);
+  if (regular) {
+    if constexpr (W > 0) {
+      const uint32_t ib = i0 + threadIdx.x * P;
+      if (ib >= i1)
+        return;
+      const uint64_t s = s0 + (uint64_t)W * (ib - i0);
+      const uint32_t cnt = min(P, i1 - ib);
+      constexpr int NIN = W * P * sizeof(T), NOUT = P * sizeof(T);
+      for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
+        const T* v = values + src_beg + (uint64_t)b * src_n + s;
+        T* o = values + dst_beg + (uint64_t)b * dst_n + ib;
+        if (cnt == P && ((uintptr_t)v & 15) == 0 && ((uintptr_t)o & (min(NOUT, 16) - 1)) == 0) {
+          T x[W * P], y[P];
+          copy_bytes<NIN>(x, v);
+#pragma unroll
+          for (uint32_t p = 0; p < P; ++p)
+            y[p] = reduce_fixed<T, M, W>(x + p * W);
+          copy_bytes<NOUT>(o, y);
+        } else {
+          for (uint32_t p = 0; p < cnt; ++p) {
+            T x[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+              x[j] = v[p * W + j];
+            o[p] = reduce_fixed<T, M, W>(x);
+          }
+        }
+      }
+    }
     return;
-  const uint64_t b = g / dst_n, i = g - b * dst_n;
-  const uint64_t s = i ? ends[i - 1] : 0, e = ends[i];
-  values[dst_beg + b * dst_n + i] = reduce_window<T, M>(values + src_beg + b * src_n, s, e);
+  }
+#pragma unroll
+  for (uint32_t p = 0; p < P; ++p) {
+    const uint32_t i = i0 + p * 256 + threadIdx.x;
+    if (i >= i1)
+      break;
+    const uint64_t s = i ? ends[i - 1] : 0;
+    const uint32_t len = (uint32_t)(ends[i] - s);
+    for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y)
+      values[dst_beg + (uint64_t)b * dst_n + i] =
+          reduce_run<T, M>(values + src_beg + (uint64_t)b * src_n + s, len);
+  }
 }
 
 template<typename T>
@@ -171,32 +441,97 @@ __global__ __launch_bounds__(256) void emit_k(T* __restrict__ dst, const T* __re
   dst[g] = accum[g];
 }
 
-template<typename T>
-__global__ __launch_bounds__(256) void scatter_k(T* __restrict__ pool, const T* __restrict__ morton,
-                                                 const uint32_t* __restrict__ lut,
-                                                 const uint64_t* __restrict__ fixed, uint64_t n_l,
-                                                 uint64_t total)
-{
-  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= total)
-    return;
-  const uint64_t b = g / n_l, m = g - b * n_l;
-  pool[fixed[b] + lut[m]] = morton[g];
-}
-
-// One work-item per element; a dispatch counts work-items in 32 bits, which
-// lod_init checks against the largest per-epoch launch.
+// Element-wise kernels count work-items in 32 bits, which lod_init checks
+// against the largest per-epoch launch.
 inline dim3 grid_of(uint64_t n) { return dim3((uint32_t)((n + 255) / 256)); }
 
-template<typename T>
-hipError_t reduce_launch(int method, T* values, uint64_t sb, uint64_t db, const uint64_t* ends,
-                         uint64_t sn, uint64_t dn, uint64_t total, hipStream_t st)
+inline uint32_t batch_grid(uint64_t nb) { return (uint32_t)std::min<uint64_t>(nb, 65535); }
+
+// A launch that records the timing events it is given: `start` when the
+// kernel begins, `stop` when it ends (either may be null).
+template<typename K, typename... A>
+void launch(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t start, hipEvent_t stop, A... args)
 {
+  if (start || stop)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, start, stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
+TileGeom geom_of(const LodTiles& t)
+{
+  // row-major order puts the coordinate with the smallest stride fastest
+  int order[3] = { 0, 1, 2 };
+  std::stable_sort(order, order + t.ndim, [&](int a, int b) {
+    return std::llabs(t.stride[a]) > std::llabs(t.stride[b]);
+  });
+  TileGeom g{};
+  for (int q = 0; q < t.ndim; ++q) {
+    g.shift[order[q]] = (uint32_t)(t.side_log * (t.ndim - 1 - q));
+    g.stride[q] = t.stride[order[q]];
+  }
+  return g;
+}
+
+template<size_t N> struct Bits;
+template<> struct Bits<1> { using type = uint8_t; };
+template<> struct Bits<2> { using type = uint16_t; };
+template<> struct Bits<4> { using type = uint32_t; };
+template<> struct Bits<8> { using type = uint64_t; };
+
+// Move one level between its Morton run (n cells per batch) and the array.
+// `start`/`stop` are recorded by the first/last kernel launched.
+template<typename T, bool ToMorton>
+hipError_t move_launch(const LodTiles& t, const uint32_t* d_tiles, const uint32_t* d_edge, T* morton_t,
+                       T* array_t, const uint64_t* fixed, uint64_t n, uint64_t nb, hipStream_t st,
+                       hipEvent_t start = nullptr, hipEvent_t stop = nullptr)
+{
+  using U = typename Bits<sizeof(T)>::type;
+  U* morton = (U*)morton_t;
+  U* array = (U*)array_t;
+  if (nb == 0)
+    return hipSuccess;
+  const uint32_t ntiles = (uint32_t)(t.tiles.size() / 2), nedge = (uint32_t)(t.edge.size() / 2);
+  const uint32_t gy = batch_grid(nb);
+  if (ntiles) {
+    const TileGeom g = geom_of(t);
+    const bool vec = t.vec && ((uintptr_t)array & 15) == 0;
+    hipEvent_t e1 = nedge ? nullptr : stop;
+    switch (t.ndim) {
+#define K(D)                                                                                     \
+  case D:                                                                                        \
+    if (vec)                                                                                     \
+      launch(tiles_k<U, D, ToMorton, true>, dim3(ntiles, gy), dim3(256), st, start, e1, morton,  \
+             array, d_tiles, fixed, n, (uint32_t)nb, g);                                         \
+    else                                                                                         \
+      launch(tiles_k<U, D, ToMorton, false>, dim3(ntiles, gy), dim3(256), st, start, e1, morton, \
+             array, d_tiles, fixed, n, (uint32_t)nb, g);                                         \
+    break;
+      K(1)
+      K(2)
+      K(3)
+#undef K
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (nedge)
+    launch(edge_k<U, ToMorton>, dim3((nedge + 255) / 256, gy), dim3(256), st, ntiles ? nullptr : start,
+           stop, morton, array, d_edge, nedge, fixed, n, (uint32_t)nb);
+  return hipGetLastError();
+}
+
+template<typename T, int W>
+hipError_t reduce_launch_w(int method, T* values, uint64_t sb, uint64_t db, const uint64_t* ends,
+                           uint64_t sn, uint64_t dn, uint64_t nb, hipStream_t st, hipEvent_t start,
+                           hipEvent_t stop)
+{
+  constexpr uint32_t span = 256 * outputs_per_lane(W, sizeof(T));
+  const dim3 grid((uint32_t)((dn + span - 1) / span), batch_grid(nb));
   switch (method) {
 #define R(M)                                                                                   \
   case M:                                                                                      \
-    hipLaunchKernelGGL((reduce_k<T, M>), grid_of(total), dim3(256), 0, st, values, sb, db, ends, \
-                       sn, dn, total);                                                         \
+    launch(reduce_k<T, M, W>, grid, dim3(256), st, start, stop, values, sb, db, ends,           \
+           (uint32_t)sn, (uint32_t)dn, (uint32_t)nb);                                          \
     break;
     R(lod_reduce_mean)
     R(lod_reduce_min)
@@ -210,6 +545,23 @@ hipError_t reduce_launch(int method, T* values, uint64_t sb, uint64_t db, const 
   return hipGetLastError();
 }
 
+template<typename T>
+hipError_t reduce_launch(int ndim, int method, T* values, uint64_t sb, uint64_t db, const uint64_t* ends,
+                         uint64_t sn, uint64_t dn, uint64_t nb, hipStream_t st, hipEvent_t start,
+                         hipEvent_t stop)
+{
+  if (dn == 0 || nb == 0)
+    return hipSuccess;
+  switch (ndim) {
+    case 1: return reduce_launch_w<T, 2>(method, values, sb, db, ends, sn, dn, nb, st, start, stop);
+    case 2: return reduce_launch_w<T, 4>(method, values, sb, db, ends, sn, dn, nb, st, start, stop);
+    case 3: return reduce_launch_w<T, 8>(method, values, sb, db, ends, sn, dn, nb, st, start, stop);
+    // more LOD dims: windows of up to 2^ndim cells; the block check never
+    // passes for W = 0, so every block reads the ends table
+    default: return reduce_launch_w<T, 0>(method, values, sb, db, ends, sn, dn, nb, st, start, stop);
+  }
+}
+
 // One epoch of the LOD path for element type T.
 template<typename T>
 int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
@@ -218,25 +570,54 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
   hipStream_t st = s->st_comp;
   T* lod = (T*)s->d_lod;
   const uint64_t n0 = p.levels[0].lod_nelem;
-  const uint64_t tot0 = p.fixed_count * n0;
-  hipLaunchKernelGGL(gather_k<T>, grid_of(tot0), dim3(256), 0, st, (const T*)s->d_linear, lod,
-                     s->d_gather_lut, s->d_gather_fixed, n0, tot0);
-  if (hipGetLastError() != hipSuccess)
+  T* linear = (T*)(s->lod_src ? s->lod_src : s->d_linear);
+  s->linear_in_lod = s->lod_src != nullptr;
+  s->lod_src = nullptr;
+  const double bpe = (double)sizeof(T), nb = (double)p.fixed_count;
+  // each pass is timed by events that its first and last kernel record
+  // (start and end of execution) and credited its algorithmic bytes
+  hipEvent_t ta = nullptr, tb = nullptr;
+  if (timing_pair(&ta, &tb) ||
+      move_launch<T, true>(s->luts.gather_tiles, s->d_gather_tiles, s->d_gather_edge, lod, linear,
+                           s->d_gather_fixed, n0, p.fixed_count, st, ta, tb) != hipSuccess)
     return 1;
-  for (int l = 0; l + 1 < p.nlod; ++l) {
-    const uint64_t sn = p.levels[l].lod_nelem, dn = p.levels[l + 1].lod_nelem;
-    if (reduce_launch<T>(p.config.reduce_method, lod, p.level_span_begin(l), p.level_span_begin(l + 1),
-                         s->d_ends + p.segment_begin(l), sn, dn, p.fixed_count * dn, st) != hipSuccess)
+  timing_add(s, ta, tb, &stream_metrics::lod_gather, nb * n0 * bpe, nb * n0 * bpe);
+  int last_reduce = -1;
+  for (int l = 0; l + 1 < p.nlod; ++l)
+    if (p.levels[l + 1].lod_nelem)
+      last_reduce = l;
+  if (last_reduce >= 0) {
+    double rin = 0, rout = 0;
+    if (timing_pair(&ta, &tb))
       return 1;
+    bool first = true;
+    for (int l = 0; l + 1 < p.nlod; ++l) {
+      const uint64_t sn = p.levels[l].lod_nelem, dn = p.levels[l + 1].lod_nelem;
+      if (!dn)
+        continue;
+      if (reduce_launch<T>(p.lod_ndim, p.config.reduce_method, lod, p.level_span_begin(l),
+                           p.level_span_begin(l + 1), s->d_ends + p.segment_begin(l), sn, dn,
+                           p.fixed_count, st, first ? ta : nullptr, l == last_reduce ? tb : nullptr) != hipSuccess)
+        return 1;
+      first = false;
+      rin += nb * sn * bpe;
+      rout += nb * dn * bpe;
+    }
+    timing_add(s, ta, tb, &stream_metrics::lod_reduce, rin, rout);
   }
   uint32_t mask = p.append_downsample ? 1u : (uint32_t)((1ull << p.nlod) - 1);
-  if (p.append_downsample) {
+  if (p.append_downsample && p.nlod > 1) {
     T* acc = (T*)s->d_accum;
+    double fin = 0, fout = 0;
+    if (timing_begin(s, &ta))
+      return 1;
     for (int lv = 1; lv < p.nlod; ++lv) {
       const uint64_t n = p.fixed_count * p.levels[lv].lod_nelem;
       hipLaunchKernelGGL(fold_k<T>, grid_of(n), dim3(256), 0, st, acc + p.append_accum_offset(lv),
                          lod + p.level_span_begin(lv), n, s->append_counts[lv], lv,
                          (int)p.config.append_reduce_method);
+      fin += (s->append_counts[lv] ? 2.0 : 1.0) * n * bpe;
+      fout += (double)n * bpe;
     }
     for (int lv = 1; lv < p.nlod; ++lv) {
       s->append_counts[lv]++;
@@ -245,19 +626,38 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
         hipLaunchKernelGGL(emit_k<T>, grid_of(n), dim3(256), 0, st, lod + p.level_span_begin(lv),
                            (const T*)(acc + p.append_accum_offset(lv)), n, s->append_counts[lv],
                            (int)p.config.append_reduce_method);
+        fin += (double)n * bpe;
+        fout += (double)n * bpe;
         s->append_counts[lv] = 0;
         mask |= 1u << lv;
       }
     }
+    if (timing_end(s, ta, &stream_metrics::lod_append_fold, fin, fout))
+      return 1;
   }
   T* pool = (T*)(s->d_pool + (uint64_t)e * s->epoch_pool_bytes);
-  for (int lv = 0; lv < p.nlod; ++lv) {
-    if (!(mask & (1u << lv)))
-      continue;
-    const uint64_t n = p.levels[lv].lod_nelem, tot = p.fixed_count * n;
-    hipLaunchKernelGGL(scatter_k<T>, grid_of(tot), dim3(256), 0, st, pool,
-                       (const T*)(lod + p.level_span_begin(lv)), s->d_chunk_lut[lv],
-                       s->d_chunk_fixed[lv], n, tot);
+  double sbytes = 0;
+  int first_lv = -1, last_lv = -1;
+  for (int lv = 0; lv < p.nlod; ++lv)
+    if ((mask & (1u << lv)) && p.levels[lv].lod_nelem) {
+      if (first_lv < 0)
+        first_lv = lv;
+      last_lv = lv;
+    }
+  if (first_lv >= 0) {
+    if (timing_pair(&ta, &tb))
+      return 1;
+    for (int lv = first_lv; lv <= last_lv; ++lv) {
+      if (!(mask & (1u << lv)) || !p.levels[lv].lod_nelem)
+        continue;
+      if (move_launch<T, false>(s->luts.chunk_tiles[lv], s->d_chunk_tiles[lv], s->d_chunk_edge[lv],
+                                lod + p.level_span_begin(lv), pool, s->d_chunk_fixed[lv],
+                                p.levels[lv].lod_nelem, p.fixed_count, st, lv == first_lv ? ta : nullptr,
+                                lv == last_lv ? tb : nullptr) != hipSuccess)
+        return 1;
+      sbytes += nb * p.levels[lv].lod_nelem * bpe;
+    }
+    timing_add(s, ta, tb, &stream_metrics::lod_morton_chunk, sbytes, sbytes);
   }
   if (hipGetLastError() != hipSuccess)
     return 1;
@@ -281,9 +681,10 @@ int drain_t(tile_stream_gpu* s, uint32_t* out_mask)
                        (const T*)(acc + p.append_accum_offset(lv)), tot, s->append_counts[lv],
                        (int)p.config.append_reduce_method);
     s->append_counts[lv] = 0;
-    hipLaunchKernelGGL(scatter_k<T>, grid_of(tot), dim3(256), 0, st, (T*)s->d_pool,
-                       (const T*)(lod + p.level_span_begin(lv)), s->d_chunk_lut[lv],
-                       s->d_chunk_fixed[lv], n, tot);
+    if (move_launch<T, false>(s->luts.chunk_tiles[lv], s->d_chunk_tiles[lv], s->d_chunk_edge[lv],
+                              lod + p.level_span_begin(lv), (T*)s->d_pool, s->d_chunk_fixed[lv], n,
+                              p.fixed_count, st) != hipSuccess)
+      return 1;
     mask |= 1u << lv;
   }
   if (hipGetLastError() != hipSuccess)
@@ -363,25 +764,48 @@ int lod_init(tile_stream_gpu* s)
     if (hipMemsetAsync(acc, 0, p.append_accum_elements() * bpe, s->st_comp) != hipSuccess)
       return 1;
   }
-  if (upload(s, &s->d_gather_lut, s->luts.gather_lut) || upload(s, &s->d_gather_fixed, s->luts.gather_fixed))
+  if (upload(s, &s->d_gather_tiles, s->luts.gather_tiles.tiles) ||
+      upload(s, &s->d_gather_edge, s->luts.gather_tiles.edge) ||
+      upload(s, &s->d_gather_fixed, s->luts.gather_fixed))
     return 1;
   std::vector<uint64_t> ends(p.ends.begin(), p.ends.end());
   if (ends.empty())
     ends.push_back(0);
   if (upload(s, &s->d_ends, ends))
     return 1;
-  s->d_chunk_lut.assign(p.nlod, nullptr);
+  s->d_chunk_tiles.assign(p.nlod, nullptr);
+  s->d_chunk_edge.assign(p.nlod, nullptr);
   s->d_chunk_fixed.assign(p.nlod, nullptr);
   for (int lv = 0; lv < p.nlod; ++lv)
-    if (upload(s, &s->d_chunk_lut[lv], s->luts.chunk_lut[lv]) ||
+    if (upload(s, &s->d_chunk_tiles[lv], s->luts.chunk_tiles[lv].tiles) ||
+        upload(s, &s->d_chunk_edge[lv], s->luts.chunk_tiles[lv].edge) ||
         upload(s, &s->d_chunk_fixed[lv], s->luts.chunk_fixed[lv]))
       return 1;
+  // the rank LUTs only served to build and check the lists
+  s->luts.gather_lut = {};
+  s->luts.chunk_lut = {};
   return 0;
 }
 
 int lod_run_epoch(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
 {
 #define CALL(T) run_epoch_t<T>(s, e, out_mask)
+  DTYPE_DISPATCH(s->plan.config.dtype, CALL);
+#undef CALL
+}
+
+template<typename T>
+int restore_t(tile_stream_gpu* s)
+{
+  const Plan& p = s->plan;
+  return move_launch<T, false>(s->luts.gather_tiles, s->d_gather_tiles, s->d_gather_edge, (T*)s->d_lod,
+                               (T*)s->d_linear, s->d_gather_fixed, p.levels[0].lod_nelem, p.fixed_count,
+                               s->st_comp) != hipSuccess;
+}
+
+int lod_restore_linear(tile_stream_gpu* s)
+{
+#define CALL(T) restore_t<T>(s)
   DTYPE_DISPATCH(s->plan.config.dtype, CALL);
 #undef CALL
 }

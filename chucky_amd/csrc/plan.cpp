@@ -3,6 +3,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace chucky {
@@ -493,6 +494,92 @@ void batch_luts(const Plan& p, int lv, uint32_t active_count, const uint32_t* po
     }
 }
 
+namespace {
+
+// Tiles of a Morton mapping `lut` (rank -> offset) over the box `shape`
+// (ndim <= 3 LOD dims). `in_one_piece(origin)` says whether the tile at that
+// origin keeps one linear stride pattern (same chunk); every accepted tile is
+// checked cell by cell against the LUT, so a tile the formula gets wrong falls
+// back to the per-cell list instead of producing wrong bytes.
+template<typename OnePiece>
+void build_tiles(int ndim, size_t bpe, const uint64_t* shape, const std::vector<uint32_t>& lut,
+                 const int64_t* stride, const std::vector<uint64_t>& fixed, OnePiece in_one_piece,
+                 LodTiles* t)
+{
+  t->tiles.clear();
+  t->edge.clear();
+  t->ndim = ndim >= 1 && ndim <= 3 ? ndim : 0;
+  t->side_log = lod_tile_side_log(t->ndim, bpe);
+  t->vec = false;
+  for (int k = 0; k < 3; ++k)
+    t->stride[k] = k < ndim ? stride[k] : 0;
+  const uint64_t n = lut.size();
+  std::vector<uint8_t> covered(n, 0);
+  if (t->side_log) {
+    const uint64_t side = 1ull << t->side_log;
+    const uint64_t cells = 1ull << (t->side_log * ndim);
+    uint64_t grid[3] = { 1, 1, 1 }, ntiles = 1;
+    for (int k = 0; k < ndim; ++k) {
+      grid[k] = shape[k] / side;
+      ntiles *= grid[k];
+    }
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+      uint64_t origin[3] = { 0, 0, 0 }, r = ti;
+      for (int k = ndim - 1; k >= 0; --k) {
+        origin[k] = (r % grid[k]) * side;
+        r /= grid[k];
+      }
+      if (!in_one_piece(origin))
+        continue;
+      const uint64_t base = morton_rank(ndim, shape, origin, 0);
+      if (base + cells > n || base > 0xFFFFFFFFull)
+        continue;
+      const uint32_t off0 = lut[base];
+      bool ok = true;
+      for (uint64_t z = 0; z < cells && ok; ++z) {
+        int64_t off = off0;
+        for (int k = 0; k < ndim; ++k) {
+          uint64_t c = 0;
+          for (int j = 0; j < t->side_log; ++j)
+            c |= ((z >> (j * ndim + k)) & 1ull) << j;
+          off += (int64_t)c * stride[k];
+        }
+        ok = (int64_t)lut[base + z] == off;
+      }
+      if (!ok)
+        continue;
+      t->tiles.push_back((uint32_t)base);
+      t->tiles.push_back(off0);
+      std::fill(covered.begin() + (int64_t)base, covered.begin() + (int64_t)(base + cells), (uint8_t)1);
+    }
+  }
+  for (uint64_t m = 0; m < n; ++m)
+    if (!covered[m]) {
+      t->edge.push_back((uint32_t)m);
+      t->edge.push_back(lut[m]);
+    }
+  // 16-byte pieces on the array side: the fastest coordinate has stride 1,
+  // a row holds whole pieces, and every other step (other strides, tile
+  // origins, batch offsets) is a multiple of a piece
+  if (t->ndim && !t->tiles.empty()) {
+    const uint64_t V = 16 / bpe;
+    int fast = 0;
+    for (int k = 1; k < ndim; ++k)
+      if (std::llabs(stride[k]) < std::llabs(stride[fast]))
+        fast = k;
+    bool ok = stride[fast] == 1 && ((1ull << t->side_log) % V) == 0;
+    for (int k = 0; k < ndim && ok; ++k)
+      ok = k == fast || (uint64_t)std::llabs(stride[k]) % V == 0;
+    for (size_t i = 1; i < t->tiles.size() && ok; i += 2)
+      ok = t->tiles[i] % V == 0;
+    for (size_t b = 0; b < fixed.size() && ok; ++b)
+      ok = fixed[b] % V == 0;
+    t->vec = ok;
+  }
+}
+
+} // namespace
+
 // src/cpu/lod.cpp:182-268 (gather LUT and batch offsets), :298-359 and
 // src/cpu/pipeline.c:369-397 (chunk LUT and batch offsets).
 void build_lod_luts(const Plan& p, LodLuts* out)
@@ -566,6 +653,44 @@ void build_lod_luts(const Plan& p, LodLuts* out)
       }
       fx[b] = (uint64_t)off + p.levels[lv].chunk_offset * L.chunk_stride;
     }
+  }
+
+  // tiled forms (LOD over at most 3 dims)
+  out->chunk_tiles.assign(p.nlod, LodTiles{});
+  if (p.lod_ndim >= 1 && p.lod_ndim <= 3) {
+    int64_t gs[3] = {};
+    for (int k = 0; k < p.lod_ndim; ++k)
+      gs[k] = (int64_t)fstride[p.lod_to_dim[k]];
+    build_tiles(p.lod_ndim, p.bpe, lshape, out->gather_lut, gs, out->gather_fixed,
+                [](const uint64_t*) { return true; }, &out->gather_tiles);
+    for (int lv = 0; lv < p.nlod; ++lv) {
+      const tile_stream_layout& L = p.levels[lv].layout;
+      uint64_t ls[3], cs[3];
+      int64_t st[3];
+      for (int k = 0; k < p.lod_ndim; ++k) {
+        const int d = p.lod_to_dim[k];
+        ls[k] = p.lod_shape(lv, k);
+        cs[k] = L.lifted_shape[2 * d + 1];
+        st[k] = L.lifted_strides[2 * d + 1];
+      }
+      const int nd = p.lod_ndim;
+      const uint64_t side = 1ull << lod_tile_side_log(nd, p.bpe);
+      build_tiles(nd, p.bpe, ls, out->chunk_lut[lv], st, out->chunk_fixed[lv],
+                  [&](const uint64_t* o) {
+                    for (int k = 0; k < nd; ++k)
+                      if (o[k] / cs[k] != (o[k] + side - 1) / cs[k])
+                        return false;
+                    return true;
+                  },
+                  &out->chunk_tiles[lv]);
+    }
+  } else {
+    // no tiled path: every cell in the per-cell lists
+    build_tiles(0, p.bpe, lshape, out->gather_lut, nullptr, out->gather_fixed,
+                [](const uint64_t*) { return true; }, &out->gather_tiles);
+    for (int lv = 0; lv < p.nlod; ++lv)
+      build_tiles(0, p.bpe, lshape, out->chunk_lut[lv], nullptr, out->chunk_fixed[lv],
+                  [](const uint64_t*) { return true; }, &out->chunk_tiles[lv]);
   }
 }
 

@@ -131,12 +131,41 @@ int build_plan(const tile_stream_configuration* config, size_t codec_alignment,
 //  gather_fixed[b]      base offset of fixed-dims batch b in the linear epoch
 //  chunk_lut[lv][m]     pool offset of level-lv Morton cell m (within its batch)
 //  chunk_fixed[lv][b]   pool offset of fixed-dims batch b at level lv
+// Side (log2) of the LOD tile for `ndim` LOD dims and `bpe`-byte elements:
+// about 4-8 KiB a tile, rows of at least 16 bytes.
+constexpr int lod_tile_side_log(int ndim, size_t bpe)
+{
+  return ndim == 1 ? (bpe == 1 ? 13 : bpe == 2 ? 12 : bpe == 4 ? 11 : 10)
+       : ndim == 2 ? (bpe <= 2 ? 6 : 5)
+       : ndim == 3 ? (bpe <= 2 ? 4 : 3)
+                   : 0;
+}
+
+// Tiled form of a Morton <-> array mapping. A tile is an aligned cube of
+// side 2^side_log over the LOD dims lying inside the box (and, for the chunk
+// pool, inside one chunk): its cells hold consecutive Morton ranks in local
+// Z-order (coordinate k takes bits k, k+D, k+2D, ... of the local index) and
+// their offsets are origin + sum(c_k * stride[k]). Cells outside full tiles
+// are listed one by one.
+struct LodTiles
+{
+  int ndim = 0;                 // LOD dims (1..3); 0: no tiles (every cell in `edge`)
+  int side_log = 0;
+  int64_t stride[3] = {};       // offset step per LOD dim inside a tile
+  bool vec = false;             // array side moves in 16-byte pieces (rows contiguous
+                                // and every row start 16-byte aligned relative to the base)
+  std::vector<uint32_t> tiles;  // (first Morton rank, origin offset) pairs
+  std::vector<uint32_t> edge;   // (Morton rank, offset) pairs
+};
+
 struct LodLuts
 {
   std::vector<uint32_t> gather_lut;
   std::vector<uint64_t> gather_fixed;
   std::vector<std::vector<uint32_t>> chunk_lut;
   std::vector<std::vector<uint64_t>> chunk_fixed;
+  LodTiles gather_tiles;              // linear epoch -> level-0 Morton order
+  std::vector<LodTiles> chunk_tiles;  // level Morton order -> chunk pool, per level
 };
 void build_lod_luts(const Plan& p, LodLuts* out);
 

@@ -229,50 +229,58 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
   return 0;
 }
 
-// Scatter launches are bracketed by events; their durations are collected
+// Launches on st_comp are bracketed by events; their durations are collected
 // once the queue has passed them.
-int scatter_begin(tile_stream_gpu* s, hipEvent_t* a)
+int time_begin(tile_stream_gpu* s, hipEvent_t* a)
 {
   HIPCHECK(hipEventCreate(a));
   HIPCHECK(hipEventRecord(*a, s->st_comp));
   return 0;
 }
 
-int scatter_end(tile_stream_gpu* s, hipEvent_t a, double bytes)
+int time_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,
+             double out_bytes)
 {
   hipEvent_t b;
   HIPCHECK(hipEventCreate(&b));
   HIPCHECK(hipEventRecord(b, s->st_comp));
-  s->pending_scatter.push_back({ a, b, bytes });
+  s->pending_timing.push_back({ a, b, in_bytes, out_bytes, which });
   return 0;
 }
 
-// Records the scatter launches that have completed, keeps the others.
+int scatter_begin(tile_stream_gpu* s, hipEvent_t* a) { return time_begin(s, a); }
+
+int scatter_end(tile_stream_gpu* s, hipEvent_t a, double bytes)
+{
+  return time_end(s, a, &stream_metrics::scatter, bytes, bytes);
+}
+
+// Records the timed launches that have completed, keeps the others.
 void collect_scatter_ready(tile_stream_gpu* s)
 {
   size_t keep = 0;
-  for (size_t i = 0; i < s->pending_scatter.size(); ++i) {
-    auto& t = s->pending_scatter[i];
+  for (size_t i = 0; i < s->pending_timing.size(); ++i) {
+    auto& t = s->pending_timing[i];
     if (hipEventQuery(t.b) == hipSuccess) {
-      add_metric(s->metrics.scatter, event_ms(t.a, t.b), t.bytes, t.bytes);
+      add_metric(s->metrics.*t.which, event_ms(t.a, t.b), t.in_bytes, t.out_bytes);
       (void)hipEventDestroy(t.a);
       (void)hipEventDestroy(t.b);
     } else {
-      s->pending_scatter[keep++] = t;
+      s->pending_timing[keep++] = t;
     }
   }
-  s->pending_scatter.resize(keep);
+  s->pending_timing.resize(keep);
 }
 
 void collect_scatter(tile_stream_gpu* s)
 {
-  for (auto& t : s->pending_scatter) {
+  for (auto& t : s->pending_timing) {
     if (hipEventSynchronize(t.b) == hipSuccess)
-      add_metric(s->metrics.scatter, event_ms(t.a, t.b), t.bytes, t.bytes);
+      add_metric(s->metrics.*t.which, event_ms(t.a, t.b), t.in_bytes, t.out_bytes);
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
   }
-  s->pending_scatter.clear();
+  s->pending_timing.clear();
 }
 
 // src/cpu/pipeline.c:112-190: compress the batch, then aggregate + deliver
@@ -336,6 +344,23 @@ int flush_drain(tile_stream_gpu* s, uint32_t mask)
 
 // ---- ingest ----
 
+// d_linear is about to take (part of) an epoch. Cells of an epoch that the
+// appends do not reach keep the previous epoch's values (the reference reuses
+// its epoch buffer); when that epoch was read in place from the caller's
+// buffer, put its values back from the Morton buffer first.
+int linear_prepare(tile_stream_gpu* s)
+{
+  if (!s->linear_in_lod)
+    return 0;
+  if (lod_restore_linear(s)) {
+    set_error("lod: restoring the epoch buffer failed");
+    return 1;
+  }
+  s->linear_in_lod = false;
+  HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
+  return 0;
+}
+
 int dispatch_stage(tile_stream_gpu* s)
 {
   const size_t bytes = s->stage_fill;
@@ -349,6 +374,8 @@ int dispatch_stage(tile_stream_gpu* s)
   HIPCHECK(hipEventCreate(&b));
   HIPCHECK(hipEventRecord(a, s->st_h2d));
   if (s->plan.enable_multiscale) {
+    if (linear_prepare(s))
+      return 1;
     HIPCHECK(hipStreamWaitEvent(s->st_h2d, s->ev_linear_free, 0));
     HIPCHECK(hipMemcpyAsync(s->d_linear + (s->cursor % E) * s->plan.bpe, s->h_stage[i], bytes,
                             hipMemcpyHostToDevice, s->st_h2d));
@@ -526,7 +553,11 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
     if (DEVICE) {
       const uint64_t off = s->cursor % E;
       if (s->plan.enable_multiscale) {
-        if (hipMemcpyAsync(s->d_linear + off * bpe, src, bytes, hipMemcpyDeviceToDevice, s->st_comp) != hipSuccess)
+        // a whole epoch in one device piece feeds the LOD gather directly
+        if (off == 0 && elems == E && ((uintptr_t)src % bpe) == 0)
+          s->lod_src = src;
+        else if (linear_prepare(s) ||
+                 hipMemcpyAsync(s->d_linear + off * bpe, src, bytes, hipMemcpyDeviceToDevice, s->st_comp) != hipSuccess)
           return writer_error_at(src, end);
       } else {
         hipEvent_t ta = nullptr;
@@ -810,6 +841,29 @@ void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes)
   add_metric(m, ms, in_bytes, out_bytes);
 }
 void scatter_collect_ready(tile_stream_gpu* s) { collect_scatter_ready(s); }
+int timing_begin(tile_stream_gpu* s, hipEvent_t* a) { return time_begin(s, a); }
+int timing_pair(hipEvent_t* a, hipEvent_t* b)
+{
+  *a = *b = nullptr;
+  if (hipEventCreate(a) != hipSuccess)
+    return 1;
+  if (hipEventCreate(b) != hipSuccess) {
+    (void)hipEventDestroy(*a);
+    *a = nullptr;
+    return 1;
+  }
+  return 0;
+}
+void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric stream_metrics::*which,
+                double in_bytes, double out_bytes)
+{
+  s->pending_timing.push_back({ a, b, in_bytes, out_bytes, which });
+}
+int timing_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,
+               double out_bytes)
+{
+  return time_end(s, a, which, in_bytes, out_bytes);
+}
 } // namespace chucky
 
 extern "C" {
@@ -960,10 +1014,11 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
   if (p.enable_multiscale) {
     lod += p.levels[0].layout.epoch_elements * p.bpe;
     lod += p.level_span_ends[p.nlod - 1] * p.bpe;
+    lod += p.levels[0].lod_nelem * 8; // level-0 gather lists
     lod += p.append_accum_elements() * p.bpe;
     lod += p.ends.size() * 8;
     for (int lv = 0; lv < p.nlod; ++lv)
-      lod += p.levels[lv].lod_nelem * 4 + p.fixed_count * 8;
+      lod += p.levels[lv].lod_nelem * 8 + p.fixed_count * 8; // tile/edge lists, at most 8 B a cell
   }
   info->lod_bytes = lod;
   if (config->codec.id == CODEC_ZSTD) {

@@ -77,12 +77,16 @@ struct tile_stream_gpu
 
   // LOD (multiscale)
   uint8_t* d_linear = nullptr;
+  const uint8_t* lod_src = nullptr; // caller's device epoch read in place of d_linear (one epoch)
+  bool linear_in_lod = false;       // the last epoch was read in place: d_linear lacks its values
   uint8_t* d_lod = nullptr;     // Morton values, all levels packed
   uint8_t* d_accum = nullptr;   // append-dim accumulators, levels >= 1
-  uint32_t* d_gather_lut = nullptr;
+  uint32_t* d_gather_tiles = nullptr;  // (Morton base, offset) per full tile
+  uint32_t* d_gather_edge = nullptr;   // (Morton rank, offset) per remaining cell
   uint64_t* d_gather_fixed = nullptr;
   uint64_t* d_ends = nullptr;
-  std::vector<uint32_t*> d_chunk_lut;
+  std::vector<uint32_t*> d_chunk_tiles;
+  std::vector<uint32_t*> d_chunk_edge;
   std::vector<uint64_t*> d_chunk_fixed;
   uint32_t append_counts[LOD_MAX_LEVELS] = {};
   std::vector<uint8_t> slot_compressed; // per pool slot: compressed by a batch since the start
@@ -105,13 +109,15 @@ struct tile_stream_gpu
   uint64_t epochs_done = 0;     // epochs whose append completed
   uint64_t epochs_written = 0;  // epochs whose shards have been written
 
-  // (start, end, bytes) event pairs of scatter launches not yet accounted
+  // event pairs around st_comp launches (scatter, LOD passes) whose
+  // durations are not yet accounted, with the metric they go to
   struct PendingTiming
   {
     hipEvent_t a, b;
-    double bytes;
+    double in_bytes, out_bytes;
+    stream_metric stream_metrics::*which;
   };
-  std::vector<PendingTiming> pending_scatter;
+  std::vector<PendingTiming> pending_timing;
 
   chucky::BatchedState* batched = nullptr; // non-null: batched pipeline active
 
@@ -122,6 +128,9 @@ struct tile_stream_gpu
 namespace chucky {
 void set_error(const std::string& msg);
 int lod_run_epoch(tile_stream_gpu* s, uint32_t epoch_in_batch, uint32_t* out_mask);
+// Writes level 0 of the Morton buffer back into d_linear (the cells the
+// gather reads), on st_comp.
+int lod_restore_linear(tile_stream_gpu* s);
 int lod_drain(tile_stream_gpu* s, uint32_t* out_mask);
 int lod_init(tile_stream_gpu* s);
 void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes);
@@ -130,6 +139,17 @@ void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes);
 // frees or initial copies.
 std::recursive_mutex& graph_mutex();
 void scatter_collect_ready(tile_stream_gpu* s);
+// Brackets launches on st_comp with events; the duration is added to
+// s->metrics.*which once the queue has passed the end event.
+int timing_begin(tile_stream_gpu* s, hipEvent_t* a);
+int timing_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,
+               double out_bytes);
+// Two timing events for launches that record them themselves (the start of
+// the first and the end of the last kernel of a pass), and their hand-over
+// to the same collection once those launches are queued.
+int timing_pair(hipEvent_t* a, hipEvent_t* b);
+void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric stream_metrics::*which,
+                double in_bytes, double out_bytes);
 uint32_t compressor_slots();
 // hipHostMalloc on the NUMA node of `device` (default policy if unknown).
 hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
