@@ -23,6 +23,14 @@ shard files (tests/golden/), and against closed forms: the standard CRC-32C
 check value, the 3x5 compacted Morton table of docs/design.md, brute-force
 Z-order ranks on tiny boxes, and the level-2 fold chain 11v/32 of
 tests/test_lod.c:919-928.
+
+f16 has no CPU path in the reference; the restatement follows its GPU path
+(float accumulator for the mean, half-precision fold and emit). Pinned: every
+level of the order-only reductions (min, max, median, 2nd extremes) and
+levels 0-1 of the mean equal float16 of the f32 reference; fold and emit
+equal it on integer-valued data, where every step is exact. Otherwise the
+f16 mean beyond level 1 and the f16 fold/emit rounding are parity unpinned
+at bit level (numpy rounds a float32 result; the GPU compares within 1 ULP).
 """
 from __future__ import annotations
 
@@ -305,9 +313,15 @@ def ravel(shape, strides, idx):
 
 # ---- LOD -------------------------------------------------------------------
 
+# Widened accumulators of src/cpu/lod.cpp:54-127. f16 has no CPU path; it
+# follows the reference's GPU path (src/gpu/lod.cu:71-75: float accumulator
+# for the mean; fold a+b and emit acc/count in half precision,
+# src/gpu/lod.cu:827-845), which numpy's float16 reproduces to within 1 ULP
+# (its half arithmetic rounds a float32 result).
 _ACC = {np.dtype(np.uint8): np.uint32, np.dtype(np.uint16): np.uint32,
         np.dtype(np.uint32): np.uint64, np.dtype(np.int8): np.int32,
-        np.dtype(np.int16): np.int32, np.dtype(np.int32): np.int64}
+        np.dtype(np.int16): np.int32, np.dtype(np.int32): np.int64,
+        np.dtype(np.float16): np.float32}
 
 
 def reduce_window(vals: np.ndarray, method: int):
@@ -317,8 +331,9 @@ def reduce_window(vals: np.ndarray, method: int):
     if method == MEAN:
         acc_t = _ACC.get(dt, dt.type)
         s = acc_t(0)
-        for v in vals:
-            s = acc_t(s + acc_t(v))
+        with np.errstate(over="ignore"):  # 64-bit sums wrap as in C
+            for v in vals:
+                s = acc_t(s + acc_t(v))
         if dt.kind in "iu":
             q = int(s) // n if int(s) >= 0 else -((-int(s)) // n)  # C truncation
             return dt.type(np.array(q).astype(dt))
@@ -353,6 +368,56 @@ def reduce_window(vals: np.ndarray, method: int):
         elif v < b2:
             b2 = v
     return b2
+
+
+def reduce_windows(src: np.ndarray, ends: np.ndarray, method: int) -> np.ndarray:
+    """reduce_window over every window [ends[i-1], ends[i]) of each row of
+    src (batches x n), vectorized across windows of equal length; each
+    window is still reduced element by element in its own order, with the
+    same operations as reduce_window (src/cpu/lod.cpp:54-127)."""
+    dt = src.dtype
+    ends = np.asarray(ends, dtype=np.int64)
+    starts = np.concatenate([[0], ends[:-1]]).astype(np.int64)
+    lens = ends - starts
+    out = np.empty((src.shape[0], len(ends)), dtype=dt)
+    for n in np.unique(lens):
+        n = int(n)
+        idx = np.flatnonzero(lens == n)
+        w = src[:, starts[idx][:, None] + np.arange(n)[None, :]]  # (B, k, n)
+        if method == MEAN:
+            acc_t = _ACC.get(dt, dt.type)
+            s = np.zeros(w.shape[:2], dtype=acc_t)
+            with np.errstate(over="ignore"):
+                for j in range(n):
+                    s = (s + w[..., j].astype(acc_t)).astype(acc_t)
+            if dt.kind in "iu":
+                si = s.astype(object) if dt.itemsize == 8 else s.astype(np.int64)
+                q = np.where(si >= 0, si // n, -((-si) // n))  # C truncation
+                r = np.array(q, dtype=object).astype(dt) if dt.itemsize == 8 else q.astype(dt)
+            else:
+                r = (s / acc_t(n)).astype(dt)
+        elif method in (MIN, MAX):
+            r = w[..., 0].copy()
+            for j in range(1, n):
+                x = w[..., j]
+                r = np.where(x < r if method == MIN else x > r, x, r)
+        elif method == MEDIAN:
+            m = min(n, 16)
+            r = np.sort(w[..., :m], axis=-1, kind="stable")[..., m // 2]
+        else:
+            t1 = w[..., 0].copy()
+            t2 = w[..., 0].copy()
+            for j in range(1, n):
+                x = w[..., j]
+                if method == MAX_SUP:
+                    first, second = x >= t1, x > t2
+                else:
+                    first, second = x <= t1, x < t2
+                t2 = np.where(first, t1, np.where(second, x, t2))
+                t1 = np.where(first, x, t1)
+            r = t2
+        out[:, idx] = r
+    return out
 
 
 def add_shift(a, b, s, dt):
@@ -436,12 +501,7 @@ class LodState:
         for l in range(p.nlod - 1):
             sn, dn = p.levels[l].lod_nelem, p.levels[l + 1].lod_nelem
             src = vals[l].reshape(p.fixed_count, sn)
-            out = np.empty((p.fixed_count, dn), dtype=self.dt)
-            e = self.ends[l]
-            for b in range(p.fixed_count):
-                for i in range(dn):
-                    s0 = e[i - 1] if i else 0
-                    out[b, i] = reduce_window(src[b, s0:e[i]], method)
+            out = reduce_windows(src, np.asarray(self.ends[l][:dn]), method)
             vals.append(out.reshape(-1))
         del n0
         return vals

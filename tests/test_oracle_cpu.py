@@ -186,3 +186,83 @@ def test_drain_without_full_batch_is_empty():
     assert len(blob) == 16 * 4 + 4  # index only
     idx = blob[:-4]
     assert all(struct.unpack_from("<QQ", idx, 16 * i) == (2**64 - 1, 2**64 - 1) for i in range(4))
+
+
+# ---- f16: pinned against float16 of the f32 reference --------------------
+
+def _chunk_arrays(shards, cps_of_level, np_dtype):
+    """{(level, shard, slot): chunk values} of CODEC_NONE shards."""
+    from tests.helpers import parse_shard
+    out = {}
+    for (lv, si), blob in shards.items():
+        for k, ent in enumerate(parse_shard(blob, cps_of_level(lv))):
+            if ent is not None:
+                off, n = ent
+                out[(lv, si, k)] = np.frombuffer(blob[off:off + n], dtype=np_dtype)
+    return out
+
+
+def _cps(dims):
+    cps = O.Plan(_odims(dims), 2).cps_total
+    return lambda lv: cps[lv]
+
+
+F16_DIMS = [Dim(8, 4, 1), Dim(40, 8, 0, downsample=True), Dim(36, 8, 0, downsample=True)]
+
+
+@needs_ref
+@pytest.mark.parametrize("red", ["mean", "min", "max", "median", "max_sup", "min_sup"])
+def test_f16_restatement_matches_f32_reference(red):
+    n = 8 * 40 * 36
+    x16 = (np.random.default_rng(5).standard_normal(n) * 100).astype(np.float16)
+    cfg = ca.Config(F16_DIMS, dtype="f32", reduce=red)
+    want, _, _ = run_ref(cfg, x16.astype(np.float32), pieces=2)
+    got, _ = O.write_stream(_odims(F16_DIMS), np.float16, x16,
+                            reduce_method=ca.REDUCE_ID[red])
+    w = _chunk_arrays(want, _cps(F16_DIMS), np.float32)
+    g = _chunk_arrays(got, _cps(F16_DIMS), np.float16)
+    assert g.keys() == w.keys()
+    levels = {k[0] for k in w}
+    assert len(levels) >= 3
+    for k in w:
+        if red == "mean" and k[0] >= 2:
+            continue  # level 2 averages rounded halfs: not a function of the f32 path
+        np.testing.assert_array_equal(g[k], w[k].astype(np.float16), err_msg=str(k))
+
+
+@needs_ref
+@pytest.mark.parametrize("ared", ["mean", "min", "max"])
+def test_f16_fold_exact_on_short_fractions(ared):
+    # multiples of 256 below 2048: every sum, mean, fold and emit is a short
+    # binary fraction, exact in half precision
+    dims = [Dim(16, 2, 2, downsample=True), Dim(16, 4, 0, downsample=True),
+            Dim(12, 4, 0, downsample=True)]
+    n = 16 * 16 * 12
+    x16 = (np.random.default_rng(9).integers(0, 8, n) * 256).astype(np.float16)
+    cfg = ca.Config(dims, dtype="f32", reduce="mean", append_reduce=ared)
+    want, _, _ = run_ref(cfg, x16.astype(np.float32), pieces=1)
+    got, _ = O.write_stream(_odims(dims), np.float16, x16, reduce_method=O.MEAN,
+                            append_method={"mean": O.MEAN, "min": O.MIN, "max": O.MAX}[ared])
+    w = _chunk_arrays(want, _cps(dims), np.float32)
+    g = _chunk_arrays(got, _cps(dims), np.float16)
+    assert g.keys() == w.keys()
+    for k in w:
+        assert np.all(w[k] == w[k].astype(np.float16).astype(np.float32)), "f32 value not a half"
+        np.testing.assert_array_equal(g[k], w[k].astype(np.float16), err_msg=str(k))
+
+
+@pytest.mark.parametrize("dtype", ["u8", "i8", "u16", "i16", "u32", "i32", "u64", "i64",
+                                   "f16", "f32", "f64"])
+@pytest.mark.parametrize("method", range(6))
+def test_vectorized_windows_match_scalar(dtype, method):
+    # ragged windows of 1..8 cells, two batch rows, values over the full range
+    rng = np.random.default_rng(31 + method)
+    lens = rng.integers(1, 9, 60)
+    ends = np.cumsum(lens)
+    src = random_typed(2 * int(ends[-1]), dtype, seed=3 + method).reshape(2, -1)
+    got = O.reduce_windows(src, ends, method)
+    for b in range(2):
+        for i, e in enumerate(ends):
+            s = ends[i - 1] if i else 0
+            want = O.reduce_window(src[b, s:e], method)
+            assert got[b, i].tobytes() == np.array(want, dtype=src.dtype).tobytes(), (b, i)
