@@ -13,6 +13,7 @@
 // each lane funnel-shifts two source dwords into one destination dword, so
 // unaligned destination offsets still produce full-width stores.
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace chucky {
 namespace {
@@ -114,9 +115,11 @@ __device__ __forceinline__ void block_copy(uint8_t* out, const uint8_t* in, uint
     out[tail0 + threadIdx.x] = in[tail0 + threadIdx.x];
 }
 
-// Workgroups take items (compressed chunks) with a grid stride and copy their
-// segments in order with a running offset; constant-size sources (CODEC_NONE
-// chunks, split into fixed pieces) are walked as (item, piece) pairs instead.
+// Workgroups take (item, segment) pairs with a grid stride: a compressed
+// segment lands at its chunk's offset plus the sizes of the chunk's earlier
+// segments (summed by the workgroup), so a batch of few large chunks still
+// spreads over the whole GPU. Constant-size sources (CODEC_NONE chunks split
+// into fixed pieces) are walked as (item, piece) pairs the same way.
 // A gather into pinned host memory runs at PCIe speed, so it is given only a
 // few CUs and leaves the rest to the compressor and transpose beside it.
 __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
@@ -127,17 +130,28 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           uint8_t* __restrict__ dst)
 {
   if (src.seg_sizes) {
-    for (uint64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __shared__ uint32_t part[256 / 64];
+    const uint64_t pairs = n_items * src.nseg;
+    for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x) {
+      const uint64_t item = pr / src.nseg;
+      const uint32_t seg = (uint32_t)(pr - item * src.nseg);
       const uint32_t chunk = gather[item];
-      uint8_t* out = dst + offsets[perm[item]];
-      const uint8_t* in = src.base + (uint64_t)chunk * src.chunk_stride;
       const uint32_t* sz = src.seg_sizes + (uint64_t)chunk * src.nseg;
-      uint64_t off = 0;
-      for (uint32_t seg = 0; seg < src.nseg; ++seg) {
-        const uint32_t nb = sz[seg];
-        block_copy(out + off, in + (uint64_t)seg * src.seg_stride, nb);
-        off += nb;
-      }
+      const uint32_t nb = sz[seg];
+      if (nb == 0)
+        continue;  // uniform across the workgroup: no barrier is skipped by part of it
+      // a chunk's compressed bytes (below its slot stride) fit 32 bits
+      uint32_t before = 0;
+      for (uint32_t j = threadIdx.x; j < seg; j += 256)
+        before += sz[j];
+      before = wave_sum(before);
+      if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = before;
+      __syncthreads();
+      before = part[0] + part[1] + part[2] + part[3];
+      __syncthreads();
+      block_copy(dst + offsets[perm[item]] + before,
+                 src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nb);
     }
     return;
   }
@@ -173,7 +187,7 @@ hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gat
 {
   if (n_items == 0)
     return hipSuccess;
-  uint64_t wg = src.seg_sizes ? n_items : n_items * src.nseg;
+  uint64_t wg = n_items * src.nseg;
   const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
   if (wg > cap)
     wg = cap;

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
         for (uint32_t i = 0; i < kPieceIters; ++i) {
           const uint32_t u = i * 256 + threadIdx.x;
           if (pieces % 256 == 0 || u < pieces)
-            *(uint4*)&buf[at(u * V)] = *(const uint4*)(a + row_offset<D, S>(u * V, g));
+            *(u32x4*)&buf[at(u * V)] = *(const u32x4*)(a + row_offset<D, S>(u * V, g));
         }
       } else {
 #pragma unroll
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
             e[j] = buf[zr + jr[j]];
           u32x4 w;
           __builtin_memcpy(&w, e, 16);
-          // streamed out: nontemporal stores measured faster for this pass
-          __builtin_nontemporal_store(w, (u32x4*)(m + u * V));
+          // plain stores: the reduce reads level 0 next, from the caches
+          *(u32x4*)(m + u * V) = w;
         }
       } else {
 #pragma unroll
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
           if (pieces % 256 != 0 && u >= pieces)
             break;
           const uint32_t zr = at(z_to_row<D, S>(u * V, g));
-          const uint4 w = *(const uint4*)(m + u * V);
+          const u32x4 w = *(const u32x4*)(m + u * V);
           U e[V];
           __builtin_memcpy(e, &w, 16);
 #pragma unroll
@@ -182,6 +182,8 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
 #pragma unroll
         for (uint32_t i = 0; i < kPieceIters; ++i) {
           const uint32_t u = i * 256 + threadIdx.x;
+          // the chunk pool is read back only by the compressor / aggregate,
+          // later: streamed out with nontemporal stores (measured faster)
           if (pieces % 256 == 0 || u < pieces)
             __builtin_nontemporal_store(*(const u32x4*)&buf[at(u * V)], (u32x4*)(a + row_offset<D, S>(u * V, g)));
         }

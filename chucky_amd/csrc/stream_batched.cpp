@@ -48,6 +48,13 @@ struct BatchedState
 {
   hipStream_t st_agg = nullptr;
   hipStream_t st_copy = nullptr;
+  // Compression queues: when one epoch's blocks fill only part of the
+  // compressor's slots, successive epochs compress side by side on NZ queues,
+  // each with its own work counter and share of the worker scratch.
+  uint32_t NZ = 1;
+  uint32_t zip_workers = 0;
+  std::vector<hipStream_t> st_zip;
+  uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
   bool dma = true; // gathered bytes reach host memory by DMA copies
   uint32_t R = 0; // host slots
   struct HostSlot
@@ -63,6 +70,7 @@ struct BatchedState
     std::vector<uint32_t*> d_gather, d_perm;
     std::vector<uint64_t*> d_sizes, d_offsets;
     hipGraphExec_t compress_exec = nullptr;
+    hipEvent_t ev_filled = nullptr;     // scatter / LOD of the slot's epoch done
     hipEvent_t ev_compressed = nullptr; // compress of the slot's epoch done
     hipEvent_t ev_gathered = nullptr;   // gather of the slot's epoch done
   };
@@ -155,6 +163,8 @@ hipEvent_t take_timing_event(BatchedState* b)
 
 int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
 {
+  const BatchedState* b = s->batched;
+  const uint32_t z = slot % b->NZ;
   const auto codec = s->plan.config.codec;
   const uint64_t T = s->plan.total_chunks;
   if (codec.id == CODEC_LZ4_NON_STANDARD)
@@ -166,7 +176,8 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
     BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                        s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
                        s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->zblock,
-                       s->d_zscratch, s->zworkers, s->d_queue, s->d_ztables, st));
+                       (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers), b->zip_workers,
+                       b->d_queues + 16 * z, s->d_ztables, st));
   return 0;
 }
 
@@ -340,6 +351,17 @@ int batched_init(tile_stream_gpu* s)
   BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
   b->dma = s->opts.batched_d2h != 1;
   b->R = std::min<uint32_t>(K, 4);
+  b->zip_workers = s->zworkers;
+  if (p.config.codec.id == CODEC_ZSTD && !s->d_ztables) {
+    const uint64_t blocks = std::max<uint64_t>((uint64_t)p.total_chunks * s->nseg, 1);
+    b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, 4), std::max<uint64_t>(1, s->zworkers / blocks));
+    b->zip_workers = (uint32_t)std::min<uint64_t>(blocks, s->zworkers / b->NZ);
+  }
+  if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
+    return 1;
+  b->st_zip.assign(b->NZ, nullptr);
+  for (auto& z : b->st_zip)
+    BCHECK(hipStreamCreateWithFlags(&z, hipStreamNonBlocking));
   // worst-case bytes of one epoch per level
   std::vector<uint64_t> level_bytes(p.nlod);
   uint64_t total = 0;
@@ -388,6 +410,7 @@ int batched_init(tile_stream_gpu* s)
       BCHECK(hipMemcpyAsync(P.d_perm[lv], pm.data(), M * 4, hipMemcpyHostToDevice, s->st_comp));
       BCHECK(hipStreamSynchronize(s->st_comp));
     }
+    BCHECK(hipEventCreateWithFlags(&P.ev_filled, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_compressed, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
     BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
@@ -405,9 +428,14 @@ void batched_free(tile_stream_gpu* s)
     return;
   if (b->st_agg)
     (void)hipStreamSynchronize(b->st_agg);
+  for (auto z : b->st_zip)
+    if (z)
+      (void)hipStreamSynchronize(z);
   for (auto& P : b->pool) {
     if (P.compress_exec)
       (void)hipGraphExecDestroy(P.compress_exec);
+    if (P.ev_filled)
+      (void)hipEventDestroy(P.ev_filled);
     if (P.ev_compressed)
       (void)hipEventDestroy(P.ev_compressed);
     if (P.ev_gathered)
@@ -434,6 +462,9 @@ void batched_free(tile_stream_gpu* s)
     (void)hipStreamDestroy(b->st_agg);
   if (b->st_copy)
     (void)hipStreamDestroy(b->st_copy);
+  for (auto z : b->st_zip)
+    if (z)
+      (void)hipStreamDestroy(z);
   delete b;
   s->batched = nullptr;
 }
@@ -455,8 +486,15 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   auto& P = b->pool[pool_slot];
   s->slot_compressed[pool_slot] = 1;
+  // the slot's compression queue takes over once the epoch is in the pool;
   // compressed slots of this pool slot are free once its last gather is done
-  BCHECK(hipStreamWaitEvent(s->st_comp, P.ev_gathered, 0));
+  hipStream_t zs = s->st_comp;
+  if (b->NZ > 1) {
+    zs = b->st_zip[pool_slot % b->NZ];
+    BCHECK(hipEventRecord(P.ev_filled, s->st_comp));
+    BCHECK(hipStreamWaitEvent(zs, P.ev_filled, 0));
+  }
+  BCHECK(hipStreamWaitEvent(zs, P.ev_gathered, 0));
   hipEvent_t c0 = nullptr, c1 = nullptr;
   if (P.compress_exec) {
     c0 = take_timing_event(b);
@@ -465,11 +503,11 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       set_error("batched: event creation failed");
       return 1;
     }
-    BCHECK(hipEventRecord(c0, s->st_comp));
-    BCHECK(hipGraphLaunch(P.compress_exec, s->st_comp));
-    BCHECK(hipEventRecord(c1, s->st_comp));
+    BCHECK(hipEventRecord(c0, zs));
+    BCHECK(hipGraphLaunch(P.compress_exec, zs));
+    BCHECK(hipEventRecord(c1, zs));
   }
-  BCHECK(hipEventRecord(P.ev_compressed, s->st_comp));
+  BCHECK(hipEventRecord(P.ev_compressed, zs));
   BCHECK(hipStreamWaitEvent(b->st_agg, P.ev_compressed, 0));
   const auto key = std::make_tuple(pool_slot, hs, mask);
   auto it = b->agg_exec.find(key);
@@ -521,6 +559,8 @@ int batched_drain(tile_stream_gpu* s)
       for (int lv = 0; lv < s->plan.nlod; ++lv)
         s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   BCHECK(hipStreamSynchronize(s->st_comp));
+  for (auto z : b->st_zip)
+    BCHECK(hipStreamSynchronize(z));
   BCHECK(hipStreamSynchronize(b->st_agg));
   BCHECK(hipStreamSynchronize(b->st_copy));
   scatter_collect_ready(s);
