@@ -100,17 +100,26 @@ def make_input(name, shape, seed=20260403):
     return ca.DeviceBuffer.from_host(vol), vol
 
 
-def run(name, frames, steps, warmup, pipeline, host_input, options=None):
+def run(name, frames, steps, warmup, pipelines, host_input, options=None):
+    """Times `steps` steps of each pipeline on the same input. The pipelines'
+    steps alternate (in alternating order), each step bracketed by device
+    synchronizations, so that both see the same state of the machine: the
+    device-to-host copy rate of these hosts falls by about half for a minute
+    after a few seconds of full load, which would otherwise favour whichever
+    pipeline runs first."""
     import chucky_amd as ca
     cfg, shape, desc = workload(name, frames)
     data, host = make_input(name, shape)
     nbytes = int(np.prod(shape)) * 2
     if host_input and host is None:
         host = data.to_host(nbytes).view(np.uint16)
-    sink = ca.DiscardSink()
-    s = ca.GpuStream(cfg, sink, pipeline=pipeline, **(options or {}))
+    streams = {}
+    for p in pipelines:
+        sink = ca.DiscardSink()
+        streams[p] = (ca.GpuStream(cfg, sink, pipeline=1 if p == "batched" else 0, **(options or {})), sink)
 
-    def one_step():
+    def one_step(p):
+        s, sink = streams[p]
         assert s.reset(sink) == 0
         if host_input:
             assert s.append(host) == 0, ca.last_error()
@@ -119,25 +128,30 @@ def run(name, frames, steps, warmup, pipeline, host_input, options=None):
         assert s.flush() == 0, ca.last_error()
 
     for _ in range(warmup):
-        one_step()
-    b0 = sink.bytes
-    m0 = snapshot(s)
-    lat = []
-    step_ms = []
-    ca.lib.chucky_device_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ts = time.perf_counter()
-        one_step()
-        step_ms.append(round((time.perf_counter() - ts) * 1e3, 2))
-        lat.extend(s.latencies())
-    ca.lib.chucky_device_synchronize()
-    dt = time.perf_counter() - t0
-    m1 = snapshot(s)
-    res = dict(seconds=dt, bytes=nbytes, out_bytes=(sink.bytes - b0) / max(steps, 1), desc=desc,
-               metrics={k: tuple(m1[k][i] - m0[k][i] for i in range(4)) for k in m1},
-               latencies=lat, ms_per_step=dt * 1e3 / max(steps, 1), step_ms=step_ms)
-    s.close()
+        for p in pipelines:
+            one_step(p)
+    b0 = {p: streams[p][1].bytes for p in pipelines}
+    m0 = {p: snapshot(streams[p][0]) for p in pipelines}
+    lat = {p: [] for p in pipelines}
+    step_ms = {p: [] for p in pipelines}
+    for i in range(steps):
+        for p in (pipelines if i % 2 == 0 else pipelines[::-1]):
+            ca.lib.chucky_device_synchronize()
+            ts = time.perf_counter()
+            one_step(p)
+            ca.lib.chucky_device_synchronize()
+            step_ms[p].append((time.perf_counter() - ts) * 1e3)
+            lat[p].extend(streams[p][0].latencies())
+    res = {}
+    for p in pipelines:
+        s, sink = streams[p]
+        m1 = snapshot(s)
+        dt = sum(step_ms[p]) / 1e3
+        res[p] = dict(seconds=dt, bytes=nbytes, out_bytes=(sink.bytes - b0[p]) / max(steps, 1), desc=desc,
+                      metrics={k: tuple(m1[k][i] - m0[p][k][i] for i in range(4)) for k in m1},
+                      latencies=lat[p], ms_per_step=dt * 1e3 / max(steps, 1),
+                      step_ms=[round(v, 2) for v in step_ms[p]])
+        s.close()
     return res, data, cfg
 
 
@@ -370,16 +384,11 @@ def main():
         print(json.dumps(serving(args)))
         return
 
-    runs = {}
     order = ["plain", "batched"] if args.pipeline == "both" else [args.pipeline]
-    data = None
-    for p in order:
-        res, data, _ = run(args.workload, args.frames, args.steps, args.warmup,
-                           1 if p == "batched" else 0, args.host_input,
-                           {"zstd_block_bytes": args.zstd_block,
-                            "host_gather_workgroups": args.gather_wg,
-                            "batched_d2h": args.batched_d2h})
-        runs[p] = res
+    runs, data, _ = run(args.workload, args.frames, args.steps, args.warmup, order, args.host_input,
+                        {"zstd_block_bytes": args.zstd_block,
+                         "host_gather_workgroups": args.gather_wg,
+                         "batched_d2h": args.batched_d2h})
     head = runs["batched" if "batched" in runs else order[0]]
     value, head_sum = summarize(head, args.steps)
     m = head["metrics"]

@@ -48,11 +48,13 @@ constexpr uint64_t kSegSlack = 320;
 constexpr uint32_t kCarrier = 64;
 constexpr int kHashLog = 12;
 constexpr uint32_t kMinMatch = 8;
+constexpr uint32_t kLaneExt = 32; // bytes a lane extends its own match before the wave takes over
 constexpr uint32_t kMaxSeq = 8192;
 constexpr uint64_t kLitsBytes = kBlock + 256;
 constexpr uint64_t kSeqBytes = (uint64_t)kMaxSeq * sizeof(Seq);
 constexpr uint64_t kSeqOutBytes = (uint64_t)kMaxSeq * 12 + 256;
-constexpr uint64_t kWorkerScratch = (kLitsBytes + kSeqBytes + kSeqOutBytes + 255) & ~255ull;
+constexpr uint64_t kAuxBytes = (uint64_t)kMaxSeq * 4;
+constexpr uint64_t kWorkerScratch = (kLitsBytes + kSeqBytes + kSeqOutBytes + kAuxBytes + 255) & ~255ull;
 constexpr int kSymPerLane = 8; // Huffman symbols per lane per round
 // one round: <= 64 lanes * 8 symbols * 11 bits, plus < 8 carried bits
 constexpr int kStageWords = (kWave * kSymPerLane * zcore::kHufMaxBits + 8 + 31) / 32 + 4;
@@ -520,9 +522,446 @@ __device__ void huf_build_wave(Lds& L, uint32_t n)
   wave_sync();
 }
 
+// Huffman tree description (RFC 8878 4.2.1.1) of L.code into L.tree by the
+// whole wave; the same bytes as zcore::huf_write_tree. Weights and their
+// histogram come from ballots; the FSE table of the weights (table log 6) is
+// built in parallel: cell j of the symbol spread is step*j mod 64 (step odd,
+// so every cell once; cells above `high` hold the low-probability symbols),
+// state-table ranks come from ballots; the two interleaved state chains then
+// run in scalar registers over the weights with the tables one entry per
+// lane. Normalization and the table header stay on lane 0 (16 symbols).
+// Returns the description's size, 0 when no description fits (raw literals).
+__device__ uint32_t huf_write_tree_wave(Lds& L)
+{
+  const int lane = lane_id();
+  zcore::HufWork& ws = L.u.huf;
+  const int nw = L.code.max_sym; // the last symbol's weight is implied
+  const int mb = L.code.max_bits;
+  constexpr uint32_t kNone = 16;
+  uint32_t wt[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int s = k * kWave + lane;
+    const uint32_t len = s < nw ? L.code.len[s] : 0u;
+    wt[k] = s < nw ? (len ? (uint32_t)(mb + 1) - len : 0u) : kNone;
+    if (s < nw)
+      ws.wt[s] = (uint8_t)wt[k];
+  }
+  // weight histogram: lane w holds the count of weight w
+  uint32_t cnt_lane = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 16; ++w) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      c += (uint32_t)__popcll(wave_ballot(wt[k] == w));
+    if ((uint32_t)lane == w)
+      cnt_lane = c;
+  }
+  const uint64_t present = wave_ballot(lane < 16 && cnt_lane > 0);
+  const int distinct = __popcll(present);
+  const int maxw = present ? 63 - __clzll(present) : 0;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t fse_size = 0;
+  if (nw > 2 && distinct >= 2) {
+    if (lane < 16)
+      ws.wcount[lane] = cnt_lane;
+    wave_sync();
+    if (lane == 0)
+      zcore::fse_normalize(ws.wnorm, ws.wcount, maxw, (uint32_t)nw, 6);
+    wave_sync();
+    const bool sym_lane = lane <= maxw;
+    const int nrm = sym_lane ? ws.wnorm[lane] : 0;
+    // cumul[s]: cells before symbol s (a low-probability symbol takes one)
+    const uint32_t cells = nrm == -1 ? 1u : (nrm > 0 ? (uint32_t)nrm : 0u);
+    const uint32_t cum = wave_incl_scan(cells) - cells;
+    const uint32_t nc = nrm > 0 ? (uint32_t)nrm : 0u;
+    const uint32_t n_incl = wave_incl_scan(nc), n_excl = n_incl - nc;
+    const uint64_t lowm = wave_ballot(sym_lane && nrm == -1);
+    const uint32_t high = 63u - (uint32_t)__popcll(lowm);
+    // spread: lane j takes the j-th position of the stepping sequence
+    const uint32_t pos = ((uint32_t)lane * 43u) & 63u; // step = 32 + 8 + 3
+    const uint64_t validm = wave_ballot(pos <= high);
+    if (pos <= high) {
+      const uint32_t t = (uint32_t)__popcll(validm & below);
+      uint32_t s = 0;
+      for (int q = 0; q <= maxw; ++q) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)n_excl, q);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)n_incl, q);
+        if (t >= lo && t < hi)
+          s = (uint32_t)q;
+      }
+      ws.spread[pos] = (uint8_t)s;
+    }
+    if ((uint32_t)lane > high) {
+      // cells from the top down take the low-probability symbols in order
+      uint64_t m = lowm;
+      for (uint32_t r = 63u - (uint32_t)lane; r > 0; --r)
+        m &= m - 1;
+      ws.spread[lane] = (uint8_t)(__ffsll((unsigned long long)m) - 1);
+    }
+    wave_sync();
+    // state table: cell u is entry cumul[s] + (cells of s before u)
+    const uint32_t su = ws.spread[lane];
+    uint32_t rank = 0;
+    for (int q = 0; q <= maxw; ++q) {
+      const uint64_t m = wave_ballot(su == (uint32_t)q);
+      if (su == (uint32_t)q)
+        rank = (uint32_t)__popcll(m & below);
+    }
+    const uint32_t base = (uint32_t)__shfl((int)cum, (int)su, kWave);
+    wave_sync();
+    ws.ct.state[base + rank] = (uint16_t)(64u + (uint32_t)lane);
+    // symbol transforms, lane s
+    uint32_t dnb = 0;
+    int dfs = 0;
+    if (nrm == 0) {
+      dnb = (7u << 16) - 64u;
+    } else if (nrm == -1 || nrm == 1) {
+      dnb = (6u << 16) - 64u;
+      dfs = (int)cum - 1;
+    } else if (nrm > 1) {
+      const uint32_t mbo = 6u - (uint32_t)zcore::highbit32((uint32_t)nrm - 1);
+      dnb = (mbo << 16) - ((uint32_t)nrm << mbo);
+      dfs = (int)cum - nrm;
+    }
+    const uint32_t symv = dnb | ((uint32_t)(dfs + 128) << 20);
+    wave_sync();
+    const uint32_t stv = ws.ct.state[lane];
+    uint32_t hsz = 0;
+    if (lane == 0)
+      hsz = (uint32_t)zcore::fse_write_ncount(ws.fse_buf, ws.wnorm, maxw, 6);
+    hsz = wave_bcast(hsz, 0);
+    // the weights' two-state stream, backward, even positions on state `se`
+    auto rl = [](uint32_t v, uint32_t i) -> uint32_t {
+      return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
+    };
+    auto weight = [&](int i) -> uint32_t {
+      const uint32_t v = (i >> 6) == 0 ? wt[0] : (i >> 6) == 1 ? wt[1] : (i >> 6) == 2 ? wt[2] : wt[3];
+      return rl(v, (uint32_t)(i & 63));
+    };
+    auto init = [&](uint32_t sym) -> uint32_t {
+      const uint32_t e = rl(symv, sym);
+      const uint32_t d = e & 0xFFFFFu;
+      const int f = (int)(e >> 20) - 128;
+      const uint32_t nb = (d + (1u << 15)) >> 16;
+      const uint32_t v = (nb << 16) - d;
+      return rl(stv, (uint32_t)((int)(v >> nb) + f));
+    };
+    uint8_t* dst = ws.fse_buf + hsz;
+    const uint32_t cap = 127u - hsz; // a description must stay below 128 bytes
+    uint64_t acc = 0;
+    uint32_t nacc = 0, nout = 0;
+    bool over = false;
+    auto put = [&](uint32_t v, uint32_t nb) {
+      acc |= (uint64_t)(v & ((1u << nb) - 1u)) << nacc;
+      nacc += nb;
+      while (nacc >= 8) {
+        if (nout >= cap)
+          over = true;
+        else if (lane == 0)
+          dst[nout] = (uint8_t)acc;
+        ++nout;
+        acc >>= 8;
+        nacc -= 8;
+      }
+    };
+    int i = nw - 1;
+    uint32_t se, so;
+    if (i & 1) {
+      so = init(weight(i));
+      se = init(weight(i - 1));
+    } else {
+      se = init(weight(i));
+      so = init(weight(i - 1));
+    }
+    for (i -= 2; i >= 0; --i) {
+      uint32_t& st = (i & 1) ? so : se;
+      const uint32_t e = rl(symv, weight(i));
+      const uint32_t d = e & 0xFFFFFu;
+      const int f = (int)(e >> 20) - 128;
+      const uint32_t nb = (st + d) >> 16;
+      put(st, nb);
+      st = rl(stv, (uint32_t)((int)(st >> nb) + f));
+    }
+    put(so, 6);
+    put(se, 6);
+    put(1, 1);
+    if (nacc) {
+      if (nout >= cap)
+        over = true;
+      else if (lane == 0)
+        dst[nout] = (uint8_t)acc;
+      ++nout;
+    }
+    if (!over)
+      fse_size = hsz + nout;
+  }
+  wave_sync();
+  const bool direct_ok = nw <= 128;
+  const uint32_t direct_size = 1u + (uint32_t)(nw + 1) / 2;
+  if (fse_size > 1 && fse_size < 128 && (!direct_ok || fse_size + 1 < direct_size)) {
+    if (lane == 0)
+      L.tree[0] = (uint8_t)fse_size;
+    for (uint32_t k = lane; k < fse_size; k += kWave)
+      L.tree[1 + k] = ws.fse_buf[k];
+    wave_sync();
+    return fse_size + 1;
+  }
+  if (!direct_ok)
+    return 0;
+  if (lane == 0)
+    L.tree[0] = (uint8_t)(127 + nw);
+  for (int k = lane; 2 * k < nw; k += kWave) {
+    const uint32_t a = ws.wt[2 * k], b = 2 * k + 1 < nw ? ws.wt[2 * k + 1] : 0u;
+    L.tree[1 + k] = (uint8_t)((a << 4) | b);
+  }
+  wave_sync();
+  return direct_size;
+}
+
+// The block's literals (every byte outside the matches, in order) into lits,
+// with their byte histogram: the matches mark their bytes in a bitmap (aux,
+// one bit per block byte), then each round the wave compacts 256 input bytes
+// by a prefix sum of the unmarked counts.
+__device__ void gather_literals(const uint8_t* in, uint32_t len, const Seq* seqs, uint32_t nseq,
+                                uint32_t* bitmap, uint8_t* lits, uint32_t* hist)
+{
+  const int lane = lane_id();
+  const uint32_t words = (len + 31) / 32;
+  for (uint32_t w = lane; w < words; w += kWave)
+    bitmap[w] = 0;
+  __threadfence_block();
+  wave_sync();
+  for (uint32_t i = lane; i < nseq; i += kWave) {
+    const Seq s = seqs[i];
+    uint32_t a = s.pos;
+    const uint32_t e = s.pos + s.ml;
+    while (a < e) {
+      const uint32_t w = a >> 5, bit = a & 31;
+      const uint32_t n = min(32u - bit, e - a);
+      const uint32_t m = (n == 32 ? ~0u : ((1u << n) - 1u)) << bit;
+      if (m == ~0u)
+        bitmap[w] = m; // a whole word of one match is this lane's alone
+      else
+        atomicOr(&bitmap[w], m);
+      a += n;
+    }
+  }
+  __threadfence_block();
+  wave_sync();
+  uint32_t out = 0;
+  for (uint32_t base = 0; base < len; base += 4 * kWave) {
+    const uint32_t p = base + 4u * (uint32_t)lane;
+    uint32_t lit = 0, v = 0;
+    if (p < len) {
+      v = ld32u(in + p);
+      const uint32_t valid = len - p >= 4 ? 0xFu : ((1u << (len - p)) - 1u);
+      lit = ~(bitmap[p >> 5] >> (p & 31)) & valid;
+    }
+    const uint32_t cnt = (uint32_t)__popc(lit);
+    const uint32_t incl = wave_incl_scan(cnt);
+    uint32_t d = out + incl - cnt;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (lit & (1u << j)) {
+        const uint32_t byte = (v >> (8 * j)) & 0xFF;
+        lits[d++] = (uint8_t)byte;
+        atomicAdd(&hist[byte], 1u);
+      }
+    out += wave_bcast(incl, kWave - 1);
+  }
+  __threadfence_block();
+  wave_sync();
+}
+
+// Sequences section (predefined LL/ML/OF tables) by the whole wave; the same
+// bytes as zcore::write_sequences. (A) lanes compute the three codes of every
+// sequence into aux; (B) the three FSE state chains run backward over the
+// sequences in scalar registers, the tables held one entry per lane and read
+// with readlane, and aux[i] receives the state bits sequence i emits (value
+// and count per table); (C) every lane assembles one sequence's fields
+// (state bits, then the LL/ML/OF extra bits) into a 128-bit word in stream
+// order, a wave prefix sum places the words, LDS atomic-ORs assemble them,
+// complete bytes are copied out (as in huf_stream_wave); the final states
+// and the end mark close the stream.
+__device__ uint32_t write_sequences_wave(Lds& L, uint8_t* out, const Seq* seqs, uint32_t* aux,
+                                         uint32_t nseq, const zcore::Predefined& t)
+{
+  const int lane = lane_id();
+  uint32_t h;
+  if (nseq < 128)
+    h = 1;
+  else if (nseq < 0x7F00)
+    h = 2;
+  else
+    h = 3;
+  if (lane == 0) {
+    if (h == 1) {
+      out[0] = (uint8_t)nseq;
+    } else if (h == 2) {
+      out[0] = (uint8_t)((nseq >> 8) + 128);
+      out[1] = (uint8_t)nseq;
+    } else {
+      out[0] = 0xFF;
+      out[1] = (uint8_t)(nseq - 0x7F00);
+      out[2] = (uint8_t)((nseq - 0x7F00) >> 8);
+    }
+    if (nseq)
+      out[h] = 0; // LL, OF, ML: predefined
+  }
+  if (nseq == 0)
+    return h;
+  uint8_t* bs = out + h + 1;
+
+  // (A) codes
+  for (uint32_t i = lane; i < nseq; i += kWave) {
+    const Seq s = seqs[i];
+    const uint32_t llc = (uint32_t)zcore::ll_code(s.ll), mlc = (uint32_t)zcore::ml_code(s.ml - 3);
+    const uint32_t ofc = (uint32_t)zcore::highbit32(s.off + 3);
+    aux[i] = llc | (mlc << 8) | (ofc << 16);
+  }
+  // tables, one entry per lane: symbol (delta_nb_bits | (delta_find_state + 128) << 20), state
+  auto sym_of = [&](const zcore::FseCTable& ct, int n) -> uint32_t {
+    return lane < n ? ct.sym[lane].delta_nb_bits | ((uint32_t)(ct.sym[lane].delta_find_state + 128) << 20) : 0u;
+  };
+  const uint32_t ll_sym = sym_of(t.ll, 36), ml_sym = sym_of(t.ml, 53), of_sym = sym_of(t.of, 29);
+  const uint32_t ll_st = t.ll.state[lane], ml_st = t.ml.state[lane];
+  const uint32_t of_st = t.of.state[lane & 31];
+  __threadfence_block();
+  __syncthreads();
+
+  // (B) state chains
+  auto rl = [](uint32_t v, uint32_t i) -> uint32_t {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
+  };
+  auto init = [&](uint32_t symv, uint32_t stv, uint32_t code) -> uint32_t {
+    const uint32_t e = rl(symv, code);
+    const uint32_t dnb = e & 0xFFFFFu;
+    const int dfs = (int)(e >> 20) - 128;
+    const uint32_t nb = (dnb + (1u << 15)) >> 16;
+    const uint32_t v = (nb << 16) - dnb;
+    return rl(stv, (uint32_t)((int)(v >> nb) + dfs));
+  };
+  // one encode step: returns (bits << 3) | count, advances the state
+  auto step = [&](uint32_t symv, uint32_t stv, uint32_t code, uint32_t& st) -> uint32_t {
+    const uint32_t e = rl(symv, code);
+    const uint32_t dnb = e & 0xFFFFFu;
+    const int dfs = (int)(e >> 20) - 128;
+    const uint32_t nb = (st + dnb) >> 16;
+    const uint32_t bits = st & ((1u << nb) - 1u);
+    st = rl(stv, (uint32_t)((int)(st >> nb) + dfs));
+    return (bits << 3) | nb;
+  };
+  uint32_t sll, sml, sof;
+  {
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)aux[nseq - 1]);
+    sml = init(ml_sym, ml_st, (c >> 8) & 0xFF);
+    sof = init(of_sym, of_st, c >> 16);
+    sll = init(ll_sym, ll_st, c & 0xFF);
+  }
+  for (long base = (long)nseq - 2; base >= 0; base -= kWave) {
+    const long my = base - lane;
+    const uint32_t v = my >= 0 ? aux[my] : 0u;
+    uint32_t outv = 0;
+    const int cnt = base + 1 < kWave ? (int)base + 1 : kWave;
+    for (int j = 0; j < cnt; ++j) {
+      const uint32_t c = rl(v, (uint32_t)j);
+      const uint32_t fo = step(of_sym, of_st, c >> 16, sof);
+      const uint32_t fm = step(ml_sym, ml_st, (c >> 8) & 0xFF, sml);
+      const uint32_t fl = step(ll_sym, ll_st, c & 0xFF, sll);
+      if (lane == j)
+        outv = fo | (fm << 9) | (fl << 18);
+    }
+    if (my >= 0)
+      aux[my] = outv;
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  // (C) bitstream, sequences in encoding order k = 0.. (sequence nseq-1-k)
+  uint32_t* stage = L.u.enc.stage;
+  for (int i = lane; i < kStageWords; i += kWave)
+    stage[i] = 0;
+  __syncthreads();
+  uint32_t flushed = 0, rel = 0;
+  for (uint32_t k0 = 0; k0 < nseq; k0 += kWave) {
+    const uint32_t k = k0 + (uint32_t)lane;
+    unsigned __int128 bits = 0;
+    uint32_t nb = 0;
+    if (k < nseq) {
+      const uint32_t i = nseq - 1 - k;
+      const Seq s = seqs[i];
+      if (k > 0) {
+        const uint32_t sb = aux[i];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          const uint32_t field = (sb >> (9 * f)) & 0x1FF;
+          bits |= (unsigned __int128)(field >> 3) << nb;
+          nb += field & 7;
+        }
+      }
+      const int llc = zcore::ll_code(s.ll), mlc = zcore::ml_code(s.ml - 3);
+      const uint32_t ofv = s.off + 3;
+      const uint32_t ofc = (uint32_t)zcore::highbit32(ofv);
+      const uint32_t llb = zcore::ll_bits(llc), mlb = zcore::ml_bits(mlc);
+      bits |= (unsigned __int128)(s.ll & ((1u << llb) - 1u)) << nb;
+      nb += llb;
+      bits |= (unsigned __int128)((s.ml - 3) & ((1u << mlb) - 1u)) << nb;
+      nb += mlb;
+      bits |= (unsigned __int128)(ofv & (uint32_t)((1ull << ofc) - 1)) << nb;
+      nb += ofc;
+    }
+    const uint32_t incl = wave_incl_scan(nb);
+    const uint32_t total = wave_bcast(incl, kWave - 1);
+    if (nb) {
+      const uint32_t p = rel + incl - nb;
+      const unsigned __int128 sh = bits << (p & 31);
+      uint32_t* w = stage + (p >> 5);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t v = (uint32_t)(sh >> (32 * q));
+        if (v)
+          atomicOr(&w[q], v);
+      }
+    }
+    __syncthreads();
+    rel += total;
+    const uint32_t full = rel >> 3;
+    wave_copy(bs + flushed, (const uint8_t*)stage, full);
+    const uint32_t carry = ((const uint8_t*)stage)[full];
+    __syncthreads();
+    const uint32_t used_words = (rel + 31) >> 5;
+    for (uint32_t i = lane; i < used_words; i += kWave)
+      stage[i] = (i == 0) ? (carry & ((1u << (rel & 7)) - 1u)) : 0u;
+    __syncthreads();
+    flushed += full;
+    rel &= 7;
+  }
+  // final states (ML, OF, LL) and the end mark: <= 7 + 6 + 5 + 6 + 1 bits
+  if (lane == 0) {
+    uint64_t acc = stage[0] & ((1u << rel) - 1u);
+    uint32_t n = rel;
+    acc |= (uint64_t)(sml & 63u) << n;
+    n += 6;
+    acc |= (uint64_t)(sof & 31u) << n;
+    n += 5;
+    acc |= (uint64_t)(sll & 63u) << n;
+    n += 6;
+    acc |= 1ull << n;
+    n += 1;
+    for (uint32_t b = 0; b < (n + 7) / 8; ++b)
+      bs[flushed + b] = (uint8_t)(acc >> (8 * b));
+    flushed += (n + 7) / 8;
+  }
+  flushed = wave_bcast(flushed, 0);
+  __syncthreads();
+  return h + 1 + flushed;
+}
+
 // Compresses one block; returns bytes written at out (header included).
-__device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
-                                   uint8_t* lits, Seq* seqs, uint8_t* seqout,
+__device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
+                                   uint8_t* lits, Seq* seqs, uint8_t* seqout, uint32_t* aux,
                                    const zcore::Predefined& pre, const ChunkTable* tab)
 {
   const int lane = lane_id();
@@ -553,37 +992,71 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
         dist = (p - (uint32_t)L.u.table[h]) & 0xFFFFu;
       }
       const bool hit = active && dist != 0 && dist <= p && ld64u(in + p - dist) == key;
+      // every hit lane extends its own match up to kLaneExt bytes forward and
+      // up to 8 bytes backward, so one window can yield several sequences
+      // without new lookups
+      uint32_t ml_lane = 0, bx_lane = 0;
+      if (hit && p - dist >= 8) {
+        const uint64_t x = ld64u(in + p - 8) ^ ld64u(in + p - dist - 8);
+        bx_lane = x ? (uint32_t)(__builtin_clzll(x) >> 3) : 8u;
+      }
+      if (hit) {
+        const uint32_t cap = min(len - p, kLaneExt);
+        ml_lane = kMinMatch;
+        while (ml_lane < cap) {
+          const uint64_t x = ld64u(in + p + ml_lane) ^ ld64u(in + p - dist + ml_lane);
+          const uint32_t nb = x ? (uint32_t)(__builtin_ctzll(x) >> 3) : 8u;
+          ml_lane = min(ml_lane + nb, cap);
+          if (nb < 8)
+            break;
+        }
+      }
       __builtin_amdgcn_wave_barrier();
       if (active)
         L.u.table[h] = (uint16_t)p;
       __builtin_amdgcn_wave_barrier();
-      const uint64_t mask = wave_ballot(hit);
+      uint64_t mask = wave_ballot(hit);
       if (mask == 0) {
         pos += kWave * stride;
         continue;
       }
-      const int first = __ffsll((unsigned long long)mask) - 1;
-      uint32_t mp = pos + (uint32_t)first * stride;
-      uint32_t mc = mp - wave_bcast(dist, first);
-      uint32_t ml = wave_match_extend(in, mp, mc, kMinMatch, len);
-      while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
-        --mp;
-        --mc;
-        ++ml;
-      }
-      const uint32_t ll = mp - anchor;
-      copy_count(lits + nlits, in + anchor, ll, L.hist);
-      nlits += ll;
-      if (lane == 0)
-        seqs[nseq] = Seq{ ll, ml, mp - mc };
-      ++nseq;
-      pos = mp + ml;
-      anchor = pos;
-      if (lane == 0 && pos >= 2 && pos - 2 <= last_start) {
-        const uint32_t q = pos - 2;
-        L.u.table[key_hash(ld64u(in + q))] = (uint16_t)q;
-      }
-      __builtin_amdgcn_wave_barrier();
+      // greedy over the window: the first hit, then the first hit at or past
+      // the end of the previous match, and so on
+      do {
+        const int first = __ffsll((unsigned long long)mask) - 1;
+        uint32_t mp = pos + (uint32_t)first * stride;
+        // `first` is uniform: lane reads, not LDS shuffles
+        uint32_t mc = mp - (uint32_t)__builtin_amdgcn_readlane((int)dist, first);
+        uint32_t ml = (uint32_t)__builtin_amdgcn_readlane((int)ml_lane, first);
+        if (ml == kLaneExt)
+          ml = wave_match_extend(in, mp, mc, ml, len);
+        uint32_t bx = (uint32_t)__builtin_amdgcn_readlane((int)bx_lane, first);
+        bx = min(bx, mp - anchor);
+        mp -= bx;
+        mc -= bx;
+        ml += bx;
+        if (bx == 8)
+          while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
+            --mp;
+            --mc;
+            ++ml;
+          }
+        const uint32_t ll = mp - anchor;
+        nlits += ll;
+        if (lane == 0)
+          seqs[nseq] = Seq{ ll, ml, mp - mc, mp };
+        ++nseq;
+        anchor = mp + ml;
+        if (lane == 0 && anchor >= 2 && anchor - 2 <= last_start) {
+          const uint32_t q = anchor - 2;
+          L.u.table[key_hash(ld64u(in + q))] = (uint16_t)q;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // lanes whose position lies inside the match are spent
+        const uint32_t spent = stride == 1 ? anchor - pos : (anchor - pos + stride - 1) / stride;
+        mask = spent >= (uint32_t)kWave ? 0ull : mask & (~0ull << spent);
+      } while (mask && nseq < kMaxSeq);
+      pos = anchor;
     }
   }
   // one literal plus one offset-1 match to the end: every byte equals in[0]
@@ -603,7 +1076,7 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
     count_bytes(in, len, L.hist);
     lits = const_cast<uint8_t*>(in);
   } else {
-    copy_count(lits + nlits, in + anchor, len - anchor, L.hist);
+    gather_literals(in, len, seqs, nseq, aux, lits, L.hist);
   }
   nlits += len - anchor;
   __syncthreads();
@@ -657,8 +1130,11 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
     __syncthreads();
     huf_build_wave(L, used);
-    if (lane == 0)
-      L.tree_size = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
+    {
+      const uint32_t ts = huf_write_tree_wave(L);
+      if (lane == 0)
+        L.tree_size = ts;
+    }
     __syncthreads();
     uint32_t bits_local = 0;
     for (int s = lane; s < 256; s += kWave)
@@ -676,9 +1152,12 @@ __device__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint
       }
     }
   }
-  // ---- sequences section into scratch (lane 0) ----
-  if (lane == 0)
-    L.seq_size = (uint32_t)zcore::write_sequences(seqout, seqs, nseq, pre);
+  // ---- sequences section into scratch ----
+  {
+    const uint32_t ss = write_sequences_wave(L, seqout, seqs, aux, nseq, pre);
+    if (lane == 0)
+      L.seq_size = ss;
+  }
   __syncthreads();
   const uint32_t mode = L.mode, seq_size = L.seq_size;
   if (L.lit_size + seq_size >= len) {
@@ -799,8 +1278,11 @@ __global__ __launch_bounds__(256) void zstd_table_k(const uint8_t* __restrict__ 
       L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
     wave_sync();
     huf_build_wave(L, used);
-    if (lane == 0)
-      L.tree_size = (uint32_t)zcore::huf_write_tree(L.tree, L.code, L.u.huf);
+    {
+      const uint32_t ts = huf_write_tree_wave(L);
+      if (lane == 0)
+        L.tree_size = ts;
+    }
     wave_sync();
     uint64_t bits_local = 0;
     for (int s = lane; s < 256; s += kWave)
@@ -821,6 +1303,21 @@ __global__ __launch_bounds__(256) void zstd_table_k(const uint8_t* __restrict__ 
     t.tree_size = tree_size;
 }
 
+// LDS of one worker. Staged workers (blocks of at most kZstdBlockMin bytes)
+// also hold their block's input, so the parse and the literal passes read LDS
+// instead of global memory: 18.3 KiB a wavefront, 8 per CU, for the short
+// launches the smallest block size is chosen for.
+template<bool Staged> struct WorkerLds
+{
+  Lds L;
+};
+template<> struct WorkerLds<true>
+{
+  Lds L;
+  alignas(16) uint8_t in[kZstdBlockMin + 64];
+};
+
+template<bool Staged>
 __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
                                              uint32_t chunk_bytes, uint32_t block, uint32_t nseg,
                                              uint64_t total_blocks, uint8_t* __restrict__ dst,
@@ -829,12 +1326,14 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
                                              uint32_t* queue, const zcore::Predefined* __restrict__ pre,
                                              const ChunkTable* __restrict__ tables)
 {
-  __shared__ Lds L;
+  __shared__ WorkerLds<Staged> W;
+  Lds& L = W.L;
   const int lane = lane_id();
   uint8_t* ws = scratch + (uint64_t)blockIdx.x * kWorkerScratch;
   uint8_t* lits = ws;
   Seq* seqs = (Seq*)(ws + kLitsBytes);
   uint8_t* seqout = ws + kLitsBytes + kSeqBytes;
+  uint32_t* aux = (uint32_t*)(ws + kLitsBytes + kSeqBytes + kSeqOutBytes);
   uint64_t item = blockIdx.x;
   for (;;) {
     if (queue) {
@@ -864,8 +1363,16 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
         skip = kCarrier;
       }
     }
-    const uint32_t sz = compress_block(L, in + skip, len - skip, out + hdr, b + 1 == nseg, lits, seqs,
-                                       seqout, *pre, tab);
+    const uint8_t* blk = in;
+    if constexpr (Staged) {
+      // block starts are 16-byte aligned (chunk strides and block sizes are)
+      for (uint32_t o = 16u * (uint32_t)lane; o < len; o += 16u * kWave)
+        *(uint4*)&W.in[o] = *(const uint4*)(in + o);
+      __syncthreads();
+      blk = W.in;
+    }
+    const uint32_t sz = compress_block(L, blk + skip, len - skip, out + hdr, b + 1 == nseg, lits, seqs,
+                                       seqout, aux, *pre, tab);
     if (lane == 0)
       seg_sizes[chunk * nseg + b] = hdr + sz;
     __syncthreads();
@@ -964,16 +1471,20 @@ hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_b
   }
   if (queue_counter) {
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(1), 0, stream, queue_counter);
-    hipLaunchKernelGGL(zstd_k, dim3(workers), dim3(64), 0, stream, src, src_stride,
-                       (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, queue_counter, pre, tabs);
-  } else {
-    if (total > workers)
-      return hipErrorInvalidValue; // one scratch slice per block in this mode
-    hipLaunchKernelGGL(zstd_k, dim3((uint32_t)total), dim3(64), 0, stream, src, src_stride,
-                       (uint32_t)chunk_bytes, block, nseg, total, dst, chunk_dst_stride, seg_sizes,
-                       (uint8_t*)scratch, nullptr, pre, tabs);
+  } else if (total > workers) {
+    return hipErrorInvalidValue; // one scratch slice per block in this mode
   }
+  const dim3 grid(queue_counter ? workers : (uint32_t)total);
+  // staged workers need 16-byte aligned block starts
+  const bool staged = block <= kZstdBlockMin && (src_stride % 16) == 0 && ((uintptr_t)src % 16) == 0;
+  if (staged)
+    hipLaunchKernelGGL(zstd_k<true>, grid, dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
+                       block, nseg, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
+                       queue_counter, pre, tabs);
+  else
+    hipLaunchKernelGGL(zstd_k<false>, grid, dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
+                       block, nseg, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
+                       queue_counter, pre, tabs);
   return hipGetLastError();
 }
 

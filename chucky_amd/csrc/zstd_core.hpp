@@ -391,6 +391,7 @@ struct Seq
   uint32_t ll;  // literals before the match
   uint32_t ml;  // match length (>= 3)
   uint32_t off; // match distance (>= 1)
+  uint32_t pos; // match start in the block (device parser; unused by the encoder)
 };
 
 // Sequences section with predefined tables. Returns bytes written.
