@@ -70,10 +70,19 @@ __device__ __forceinline__ void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Value of lane `src_lane` (a wave-uniform index) for every lane, as a scalar:
+// code that follows it stays uniform (scalar registers, no exec-mask flow).
 template<typename T>
 __device__ __forceinline__ T wave_bcast(T v, int src_lane)
 {
-  return __shfl(v, src_lane, kWave);
+  static_assert(sizeof(T) == 4, "32-bit values");
+  return (T)__builtin_amdgcn_readlane((int)v, src_lane);
+}
+
+// A value every lane holds alike, made visibly uniform to the compiler.
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // Inclusive prefix sum over the wave.

@@ -535,8 +535,8 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
 {
   const int lane = lane_id();
   zcore::HufWork& ws = L.u.huf;
-  const int nw = L.code.max_sym; // the last symbol's weight is implied
-  const int mb = L.code.max_bits;
+  const int nw = (int)uniform((uint32_t)L.code.max_sym); // the last symbol's weight is implied
+  const int mb = (int)uniform((uint32_t)L.code.max_bits);
   constexpr uint32_t kNone = 16;
   uint32_t wt[4];
 #pragma unroll
@@ -632,7 +632,9 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
     if (lane == 0)
       hsz = (uint32_t)zcore::fse_write_ncount(ws.fse_buf, ws.wnorm, maxw, 6);
     hsz = wave_bcast(hsz, 0);
-    // the weights' two-state stream, backward, even positions on state `se`
+    // the weights' two-state stream, backward, even positions on state `se`:
+    // the state chains run in scalar registers and record each step's
+    // (bits, count) in LDS; the wave then packs the records in parallel
     auto rl = [](uint32_t v, uint32_t i) -> uint32_t {
       return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
     };
@@ -648,24 +650,7 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
       const uint32_t v = (nb << 16) - d;
       return rl(stv, (uint32_t)((int)(v >> nb) + f));
     };
-    uint8_t* dst = ws.fse_buf + hsz;
-    const uint32_t cap = 127u - hsz; // a description must stay below 128 bytes
-    uint64_t acc = 0;
-    uint32_t nacc = 0, nout = 0;
-    bool over = false;
-    auto put = [&](uint32_t v, uint32_t nb) {
-      acc |= (uint64_t)(v & ((1u << nb) - 1u)) << nacc;
-      nacc += nb;
-      while (nacc >= 8) {
-        if (nout >= cap)
-          over = true;
-        else if (lane == 0)
-          dst[nout] = (uint8_t)acc;
-        ++nout;
-        acc >>= 8;
-        nacc -= 8;
-      }
-    };
+    uint16_t* rec = (uint16_t*)ws.w; // step j: (bits << 3) | count
     int i = nw - 1;
     uint32_t se, so;
     if (i & 1) {
@@ -675,24 +660,70 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
       se = init(weight(i));
       so = init(weight(i - 1));
     }
-    for (i -= 2; i >= 0; --i) {
-      uint32_t& st = (i & 1) ? so : se;
-      const uint32_t e = rl(symv, weight(i));
-      const uint32_t d = e & 0xFFFFFu;
-      const int f = (int)(e >> 20) - 128;
-      const uint32_t nb = (st + d) >> 16;
-      put(st, nb);
-      st = rl(stv, (uint32_t)((int)(st >> nb) + f));
+    i -= 2;
+    uint32_t j = 0;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+      const uint32_t wk = wt[k];
+      for (; i >= 64 * k; --i, ++j) {
+        const uint32_t sym = rl(wk, (uint32_t)(i & 63));
+        const bool odd = i & 1;
+        const uint32_t st = odd ? so : se;
+        const uint32_t e = rl(symv, sym);
+        const uint32_t d = e & 0xFFFFFu;
+        const int f = (int)(e >> 20) - 128;
+        const uint32_t nb = (st + d) >> 16;
+        if (lane == 0)
+          rec[j] = (uint16_t)(((st & ((1u << nb) - 1u)) << 3) | nb);
+        const uint32_t ns = rl(stv, (uint32_t)((int)(st >> nb) + f));
+        so = odd ? ns : so;
+        se = odd ? se : ns;
+      }
     }
-    put(so, 6);
-    put(se, 6);
-    put(1, 1);
-    if (nacc) {
-      if (nout >= cap)
-        over = true;
-      else if (lane == 0)
-        dst[nout] = (uint8_t)acc;
-      ++nout;
+    const uint32_t nsteps = j;
+    wave_sync();
+    // pack: lane l takes steps 4l .. 4l+3 (nsteps <= 253)
+    uint32_t bits = 0, nb = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t s = 4u * (uint32_t)lane + (uint32_t)q;
+      if (s < nsteps) {
+        const uint32_t r = rec[s];
+        bits |= (r >> 3) << nb;
+        nb += r & 7;
+      }
+    }
+    const uint32_t incl = wave_incl_scan(nb);
+    const uint32_t total = wave_bcast(incl, kWave - 1);
+    uint32_t* stage = (uint32_t*)ws.spread; // 128 words
+    for (int q = lane; q < 128; q += kWave)
+      stage[q] = 0;
+    wave_sync();
+    if (nb) {
+      const uint32_t p = incl - nb;
+      const uint64_t sh = (uint64_t)bits << (p & 31);
+      atomicOr(&stage[p >> 5], (uint32_t)sh);
+      if ((uint32_t)(sh >> 32))
+        atomicOr(&stage[(p >> 5) + 1], (uint32_t)(sh >> 32));
+    }
+    wave_sync();
+    if (lane == 0) {
+      // final states and the end mark
+      uint64_t tail = (uint64_t)(so & 63u) | ((uint64_t)(se & 63u) << 6) | (1ull << 12);
+      const uint32_t p = total;
+      tail <<= (p & 31);
+      stage[p >> 5] |= (uint32_t)tail;
+      if ((uint32_t)(tail >> 32))
+        stage[(p >> 5) + 1] |= (uint32_t)(tail >> 32);
+    }
+    wave_sync();
+    const uint32_t nout = (total + 13 + 7) / 8;
+    const uint32_t cap = 127u - hsz; // a description must stay below 128 bytes
+    const bool over = nout > cap;
+    if (!over) {
+      uint8_t* dst = ws.fse_buf + hsz;
+      for (uint32_t q = lane; q < nout; q += kWave)
+        dst[q] = ((const uint8_t*)stage)[q];
     }
     if (!over)
       fse_size = hsz + nout;
@@ -786,9 +817,10 @@ __device__ void gather_literals(const uint8_t* in, uint32_t len, const Seq* seqs
 // complete bytes are copied out (as in huf_stream_wave); the final states
 // and the end mark close the stream.
 __device__ uint32_t write_sequences_wave(Lds& L, uint8_t* out, const Seq* seqs, uint32_t* aux,
-                                         uint32_t nseq, const zcore::Predefined& t)
+                                         uint32_t nseq_in, const zcore::Predefined& t)
 {
   const int lane = lane_id();
+  const uint32_t nseq = uniform(nseq_in);
   uint32_t h;
   if (nseq < 128)
     h = 1;
