@@ -11,17 +11,15 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
-// 4 bytes at any address, from two aligned dword loads. Reads up to 3 bytes
-// past p+4: every codec input buffer carries >= 64 bytes of slack.
+// 4 bytes at any address, from two aligned dword loads funnel-shifted (no
+// branch on the alignment, so lanes never diverge). Reads up to 7 bytes past
+// p: every codec input buffer carries >= 64 bytes of slack.
 __device__ __forceinline__ uint32_t ld32u(const uint8_t* p)
 {
   const uintptr_t a = (uintptr_t)p;
   const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
   const uint32_t sh = (uint32_t)(a & 3) * 8;
-  const uint32_t lo = w[0];
-  if (sh == 0)
-    return lo;
-  return (uint32_t)((((uint64_t)w[1] << 32) | lo) >> sh);
+  return __builtin_amdgcn_alignbit(w[1], w[0], sh);
 }
 
 // 8 bytes at any address, from three aligned dword loads (same slack rule).
@@ -30,12 +28,9 @@ __device__ __forceinline__ uint64_t ld64u(const uint8_t* p)
   const uintptr_t a = (uintptr_t)p;
   const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
   const uint32_t sh = (uint32_t)(a & 3) * 8;
-  const uint32_t w0 = w[0], w1 = w[1];
-  if (sh == 0)
-    return ((uint64_t)w1 << 32) | w0;
-  const uint32_t w2 = w[2];
-  const uint32_t lo = (uint32_t)((((uint64_t)w1 << 32) | w0) >> sh);
-  const uint32_t hi = (uint32_t)((((uint64_t)w2 << 32) | w1) >> sh);
+  const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+  const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, sh);
+  const uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, sh);
   return ((uint64_t)hi << 32) | lo;
 }
 

@@ -1079,9 +1079,16 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
           seqs[nseq] = Seq{ ll, ml, mp - mc, mp };
         ++nseq;
         anchor = mp + ml;
-        if (lane == 0 && anchor >= 2 && anchor - 2 <= last_start) {
-          const uint32_t q = anchor - 2;
-          L.u.table[key_hash(ld64u(in + q))] = (uint16_t)q;
+        if (anchor >= 2 && anchor - 2 <= last_start) {
+          // position q's hash is at hand when a lane of this window holds q
+          const uint32_t q = anchor - 2, dq = q - pos;
+          uint32_t hq;
+          if (stride == 1 && q >= pos && dq < (uint32_t)kWave)
+            hq = (uint32_t)__builtin_amdgcn_readlane((int)h, (int)dq);
+          else
+            hq = key_hash(ld64u(in + q));
+          if (lane == 0)
+            L.u.table[hq] = (uint16_t)q;
         }
         __builtin_amdgcn_wave_barrier();
         // lanes whose position lies inside the match are spent
