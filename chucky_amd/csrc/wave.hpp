@@ -16,18 +16,20 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 // p: every codec input buffer carries >= 64 bytes of slack.
 __device__ __forceinline__ uint32_t ld32u(const uint8_t* p)
 {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
-  const uint32_t sh = (uint32_t)(a & 3) * 8;
-  return __builtin_amdgcn_alignbit(w[1], w[0], sh);
+  // stepping back from p (rather than masking its integer value) keeps the
+  // pointer's address space known, so LDS input compiles to ds_read and
+  // global input to global_load instead of flat loads
+  const uint32_t mis = (uint32_t)(uintptr_t)p & 3u;
+  const uint32_t* w = (const uint32_t*)__builtin_assume_aligned(p - mis, 4);
+  return __builtin_amdgcn_alignbit(w[1], w[0], mis * 8);
 }
 
 // 8 bytes at any address, from three aligned dword loads (same slack rule).
 __device__ __forceinline__ uint64_t ld64u(const uint8_t* p)
 {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
-  const uint32_t sh = (uint32_t)(a & 3) * 8;
+  const uint32_t mis = (uint32_t)(uintptr_t)p & 3u;
+  const uint32_t* w = (const uint32_t*)__builtin_assume_aligned(p - mis, 4);
+  const uint32_t sh = mis * 8;
   const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
   const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, sh);
   const uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, sh);

@@ -62,12 +62,25 @@ constexpr int kStageWords = (kWave * kSymPerLane * zcore::kHufMaxBits + 8 + 31) 
 // ~10.2 KiB per wavefront, so that 15 wavefronts fit a CU's LDS. The union
 // is the parse's hash table, then the Huffman builder's scratch, then the
 // literal encoder's packed codes and bit staging.
+// Scratch of the package-merge code-length builder; overlays HufWork and
+// keeps its symbol list (sym) in place.
+struct PmWork
+{
+  uint16_t sym[256];
+  // leaf weights ascending at [0, 512), packages at [512, 1024), each area
+  // padded with ~0u so that a binary search needs no bounds
+  uint32_t lp[1024];
+  uint32_t list[512];
+  uint32_t bits[zcore::kHufMaxBits][16]; // leaf positions in each level's list
+};
+
 struct Lds
 {
   union
   {
     uint16_t table[1 << kHashLog]; // low 16 bits of the last position per hash
     zcore::HufWork huf;
+    PmWork pm;
     struct
     {
       uint32_t cw[256]; // code value | length << 16
@@ -80,6 +93,8 @@ struct Lds
   // decisions published by lane 0
   uint32_t mode, tree_size, lit_size, lit_hsize, seq_size;
 };
+static_assert(offsetof(PmWork, sym) == offsetof(zcore::HufWork, sym), "PmWork keeps HufWork's symbol list");
+static_assert(sizeof(PmWork) <= sizeof(uint16_t) << kHashLog, "the builder fits the hash table's space");
 
 // Huffman table shared by every block of a chunk (chunk-table mode).
 struct ChunkTable
@@ -93,27 +108,6 @@ struct ChunkTable
 __device__ __forceinline__ uint32_t key_hash(uint64_t key)
 {
   return (uint32_t)((key * 0xCF1BBCDCB7A56463ull) >> (64 - kHashLog));
-}
-
-// Ascending bitonic sort of 256 keys in LDS by one wavefront (two
-// compare-exchange pairs per lane per stage).
-__device__ void wave_sort256(uint32_t* keys)
-{
-  const uint32_t lane = (uint32_t)lane_id();
-  for (uint32_t k = 2; k <= 256; k <<= 1)
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t t = lane; t < 128; t += kWave) {
-        const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-        const uint32_t l = i | j;
-        const bool asc = (i & k) == 0;
-        const uint32_t a = keys[i], b = keys[l];
-        if ((a > b) == asc) {
-          keys[i] = b;
-          keys[l] = a;
-        }
-      }
-      wave_sync();
-    }
 }
 
 // Adds every byte of src[0, n) to the LDS histogram. Each lane keeps one
@@ -184,14 +178,6 @@ __device__ __forceinline__ void count_bytes_block(const uint8_t* src, uint64_t n
       atomicAdd(&hist[sym[k]], run[k]);
   for (uint64_t i = body + threadIdx.x; i < n; i += blockDim.x)
     atomicAdd(&hist[src[i]], 1u);
-}
-
-// Compacts in[0, n) to dst and adds every byte to the LDS histogram.
-__device__ __forceinline__ void copy_count(uint8_t* dst, const uint8_t* src, uint32_t n,
-                                           uint32_t* hist)
-{
-  wave_copy(dst, src, n);
-  count_bytes(src, n, hist);
 }
 
 // The up-to-8 symbols lane `lane` encodes in the round whose highest symbol
@@ -282,200 +268,164 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
   }
 }
 
-// Huffman code lengths and canonical values for the n >= 2 used symbols in
-// L.u.huf.sym[0..n) (ascending count, ties by symbol), built by the whole
-// wave: two-queue Huffman merge and the depth pass over register-resident
-// arrays, length limiting to kHufMaxBits on per-length counts (Kraft sum
-// restored exactly), lengths handed out in count order, canonical values by
-// wave ballots. Same code as zcore::huf_build_sorted produces on the host.
-__device__ void huf_build_wave(Lds& L, uint32_t n)
+// The n >= 2 used symbols in ascending count order, ties by symbol, into
+// L.u.pm.sym[0..n) and their counts into L.u.pm.lp[0..n): a bitonic sort of
+// the 256 keys (count << 8 | symbol) held in registers, element 64r + lane in
+// register r; partners in another register are compared within the lane,
+// partners in another lane are exchanged by cross-lane shuffles. Unused
+// symbols (count 0) sort first and are dropped. Counts stay below 2^24.
+__device__ void huf_sort_wave(Lds& L, uint32_t n)
+{
+  const uint32_t lane = (uint32_t)lane_id();
+  uint32_t v[4];
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t sy = 64u * r + lane;
+    v[r] = (L.hist[sy] << 8) | sy;
+  }
+#pragma unroll
+  for (uint32_t k = 2; k <= 256; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= (uint32_t)kWave) {
+        // partner in register r ^ (j / 64); k > j >= 64, so the direction
+        // depends on the register only
+        const uint32_t rj = j / kWave;
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+          if (r & rj)
+            continue;
+          const bool asc = ((64u * r) & k) == 0;
+          const uint32_t a = v[r], b = v[r | rj];
+          const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+          v[r] = asc ? lo : hi;
+          v[r | rj] = asc ? hi : lo;
+        }
+      } else {
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+          const uint32_t y = (uint32_t)__shfl_xor((int)v[r], (int)j, kWave);
+          const bool asc = ((64u * r + lane) & k) == 0;
+          const bool keep_min = ((lane & j) == 0) == asc;
+          const uint32_t lo = v[r] < y ? v[r] : y, hi = v[r] < y ? y : v[r];
+          v[r] = keep_min ? lo : hi;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t e = 64u * r + lane;
+    if (e >= 256u - n) {
+      L.u.pm.sym[e - (256u - n)] = (uint16_t)(v[r] & 0xFF);
+      L.u.pm.lp[e - (256u - n)] = v[r] >> 8;
+    }
+  }
+  wave_sync();
+}
+
+// Huffman code lengths (at most kHufMaxBits, a complete prefix code) and
+// canonical values for the n >= 2 used symbols in L.u.pm.sym[0..n), counts in
+// L.u.pm.lp[0..n) ascending, by the package-merge algorithm (Larmore and
+// Hirschberg, "A fast algorithm for optimal length-restricted Huffman codes",
+// 1990) run by the whole wave. Level kHufMaxBits is the leaves alone; level
+// d < kHufMaxBits merges the leaves with the packages (sums of adjacent
+// pairs) of level d + 1, every item ranked in the other sorted array by a
+// binary search (ties: leaves first). The first 2n - 2 items of level 1 are
+// selected; the packages among the first m items of a level select the first
+// 2 (m - leaves) items of the next, so each level only records where its
+// leaves sit, and a leaf's code length is the number of levels that select
+// it. The code is optimal among codes of at most kHufMaxBits bits.
+__device__ void huf_build_wave(Lds& L, uint32_t n_in)
 {
   const uint32_t lane = (uint32_t)lane_id();
   constexpr uint32_t Lmax = zcore::kHufMaxBits;
-  // Two-queue Huffman merge. Leaf weights (ascending) and internal-node
-  // weights (created in ascending order) live in registers, 64 per register,
-  // lane j of register k holding entry 64k+j; the merge keeps both queue
-  // heads in scalars and refreshes only the popped head with one lane read.
-  // Parent links go to LDS (fire-and-forget stores by lane 0).
-  uint32_t lw[4], nw[4] = { 0, 0, 0, 0 };
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t idx = (uint32_t)k * kWave + lane;
-    lw[k] = idx < n ? L.hist[L.u.huf.sym[idx]] : 0u;
-  }
-  auto pick4 = [](const uint32_t (&r)[4], uint32_t i) -> uint32_t {
-    uint32_t v;
-    switch (i >> 6) {
-      case 0: v = r[0]; break;
-      case 1: v = r[1]; break;
-      case 2: v = r[2]; break;
-      default: v = r[3]; break;
-    }
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(i & 63));
-  };
-  int16_t* parent = L.u.huf.parent;
-  uint32_t leaf = 0, node = 0, made = 0;
-  uint32_t hl = pick4(lw, 0), hn = 0xFFFFFFFFu;
-  for (uint32_t it = 0; it + 1 < n; ++it) {
-    uint32_t pk0 = 0, pk1 = 0, ws = 0;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      uint32_t pk;
-      if (leaf < n && hl <= hn) { // ties go to the leaf
-        pk = leaf;
-        ws += hl;
-        ++leaf;
-        hl = leaf < n ? pick4(lw, leaf) : 0xFFFFFFFFu;
-      } else {
-        pk = n + node;
-        ws += hn;
-        ++node;
-        hn = node < made ? pick4(nw, node) : 0xFFFFFFFFu;
-      }
-      if (c == 0)
-        pk0 = pk;
-      else
-        pk1 = pk;
-    }
-    const bool mine = lane == (made & 63);
-    switch (made >> 6) {
-      case 0: nw[0] = mine ? ws : nw[0]; break;
-      case 1: nw[1] = mine ? ws : nw[1]; break;
-      case 2: nw[2] = mine ? ws : nw[2]; break;
-      default: nw[3] = mine ? ws : nw[3]; break;
-    }
-    if (lane == 0) {
-      parent[pk0] = (int16_t)(n + made);
-      parent[pk1] = (int16_t)(n + made);
-    }
-    if (node == made)
-      hn = ws; // the node queue was empty: the new node is its head
-    ++made;
-  }
-  const uint32_t nodes = n + made; // root = nodes - 1
-  if (lane == 0)
-    parent[nodes - 1] = (int16_t)(nodes - 1);
+  PmWork& pm = L.u.pm;
+  const uint32_t n = uniform(n_in);
+  for (uint32_t k = n + lane; k < 512; k += kWave)
+    pm.lp[k] = ~0u;
+  for (uint32_t k = lane; k < 512; k += kWave)
+    pm.lp[512 + k] = ~0u;
   wave_sync();
-  // depths by pointer jumping: every node adds its parent's distance to the
-  // root and jumps to its grandparent; 9 rounds cover depths < 512
-  uint8_t* dep = L.u.huf.depth;
-  uint32_t pj[8], dj[8];
+  uint32_t items = n; // size of the previous level's list
+  bool from_leaves = true;
+  for (uint32_t d = Lmax - 1; d >= 1; --d) {
+    // the package count never shrinks from level to level (it stays below
+    // n), so the padding above it is never overwritten
+    const uint32_t np = items >> 1;
+    for (uint32_t k = lane; k < np; k += kWave)
+      pm.lp[512 + k] = from_leaves ? pm.lp[2 * k] + pm.lp[2 * k + 1] : pm.list[2 * k] + pm.list[2 * k + 1];
+    if (lane < 16)
+      pm.bits[d][lane] = 0;
+    wave_sync();
+    const uint32_t total = n + np; // <= 511: at most 8 items per lane
+    uint32_t thr[8], off[8], c[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const uint32_t i = (uint32_t)k * kWave + lane;
-    pj[k] = i < nodes ? (uint32_t)parent[i] : i;
-    dj[k] = i < nodes && i != nodes - 1 ? 1u : 0u;
-  }
-  for (int round = 0; round < 9; ++round) {
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t t = lane + (uint32_t)(kWave * q);
+      const bool leaf = t < n;
+      const uint32_t w = pm.lp[leaf ? t : 512 + (t - n)];
+      // a leaf counts the packages lighter than it, a package the leaves
+      // at most as heavy (padding items are never counted)
+      thr[q] = leaf ? w : w + 1;
+      off[q] = leaf ? 512u : 0u;
+      c[q] = 0;
+    }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint32_t i = (uint32_t)k * kWave + lane;
-      if (i < nodes) {
-        dep[i] = (uint8_t)dj[k];
-        parent[i] = (int16_t)pj[k];
+    for (uint32_t step = 256; step; step >>= 1) {
+      uint32_t v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        v[q] = pm.lp[off[q] + c[q] + step - 1];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        c[q] += v[q] < thr[q] ? step : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t t = lane + (uint32_t)(kWave * q);
+      if (t < total) {
+        const bool leaf = t < n;
+        const uint32_t pos = (leaf ? t : t - n) + c[q];
+        pm.list[pos] = leaf ? thr[q] : thr[q] - 1;
+        if (leaf)
+          atomicOr(&pm.bits[d][pos >> 5], 1u << (pos & 31));
       }
     }
     wave_sync();
-    uint32_t dp[8], pp[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint32_t i = (uint32_t)k * kWave + lane;
-      dp[k] = i < nodes ? dep[pj[k]] : 0u;
-      pp[k] = i < nodes ? (uint32_t)parent[pj[k]] : pj[k];
-    }
-    wave_sync();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      dj[k] += dp[k];
-      pj[k] = pp[k];
-    }
+    items = total;
+    from_leaves = false;
   }
-  struct
-  {
-    uint32_t r[4];
-  } D;
+  // selection: m items of level d hold a_d leaves and select 2 (m - a_d)
+  // items of level d + 1; level kHufMaxBits holds leaves only
+  uint32_t a[Lmax + 1];
+  uint32_t m = 2 * n - 2;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    D.r[k] = dj[k];
-
-  // leaf depths of this lane (leaves k*64 + lane, k < 4)
+  for (uint32_t d = 1; d < Lmax; ++d) {
+    uint32_t cnt = 0;
+    if (lane < 16) {
+      const uint32_t lo = 32u * lane;
+      const uint32_t mk = m <= lo ? 0u : (m - lo >= 32 ? ~0u : (1u << (m - lo)) - 1u);
+      cnt = (uint32_t)__popc(pm.bits[d][lane] & mk);
+    }
+    a[d] = uniform(wave_sum(cnt));
+    m = 2 * (m - a[d]);
+  }
+  a[Lmax] = m;
   uint32_t len[4];
   uint32_t maxd = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint32_t idx = (uint32_t)k * kWave + lane;
-    len[k] = idx < n ? D.r[k] : 0u;
+    uint32_t l = 0;
+#pragma unroll
+    for (uint32_t d = 1; d <= Lmax; ++d)
+      l += idx < a[d] ? 1u : 0u;
+    len[k] = idx < n ? l : 0u;
     maxd = len[k] > maxd ? len[k] : maxd;
   }
-  maxd = wave_max(maxd);
-  if (maxd > Lmax) {
-    // per-length code counts live in lanes: lane b holds the count of length b
-    uint32_t blv = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t d = len[k] > Lmax ? Lmax : len[k];
-#pragma unroll
-      for (uint32_t b = 1; b <= Lmax; ++b) {
-        const uint32_t c = (uint32_t)__popcll(wave_ballot(len[k] != 0 && d == b));
-        if (lane == b)
-          blv += c;
-      }
-    }
-    const int64_t full = (int64_t)1 << Lmax;
-    int64_t kraft = (int64_t)wave_sum(lane >= 1 && lane <= Lmax ? blv << (Lmax - lane) : 0u);
-    while (kraft > full) {
-      // lengthen codes at the longest length below Lmax (smallest decrement)
-      const uint64_t nz = wave_ballot(lane >= 1 && lane < Lmax && blv != 0);
-      const uint32_t b = 63u - (uint32_t)__clzll(nz);
-      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
-      const int64_t step = (int64_t)1 << (Lmax - b - 1);
-      int64_t k = (kraft - full + step - 1) / step;
-      if (k > (int64_t)cnt)
-        k = cnt;
-      if (lane == b)
-        blv -= (uint32_t)k;
-      if (lane == b + 1)
-        blv += (uint32_t)k;
-      kraft -= k * step;
-    }
-    while (kraft < full) {
-      // shorten codes at the longest length whose gain still fits
-      const int64_t gap = full - kraft;
-      const uint64_t ok = wave_ballot(lane > 1 && lane <= Lmax && blv != 0 &&
-                                      ((int64_t)1 << (Lmax - lane)) <= gap);
-      if (!ok)
-        break;
-      const uint32_t b = 63u - (uint32_t)__clzll(ok);
-      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
-      const int64_t step = (int64_t)1 << (Lmax - b);
-      int64_t k = gap / step;
-      if (k > (int64_t)cnt)
-        k = cnt;
-      if (lane == b)
-        blv -= (uint32_t)k;
-      if (lane == b - 1)
-        blv += (uint32_t)k;
-      kraft += k * step;
-    }
-    // longest codes to the least frequent leaves: leaf ranks [above, above +
-    // count) of length b, with `above` the number of codes longer than b
-    uint32_t above = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      len[k] = 0;
-    for (uint32_t b = Lmax; b >= 1; --b) {
-      const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)blv, (int)b);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t idx = (uint32_t)k * kWave + lane;
-        if (idx < n && idx >= above && idx < above + cnt)
-          len[k] = b;
-      }
-      above += cnt;
-    }
-    const uint64_t nz = wave_ballot(lane >= 1 && lane <= Lmax && blv != 0);
-    maxd = 63u - (uint32_t)__clzll(nz);
-  }
+  maxd = uniform(wave_max(maxd));
+  wave_sync();
   // scatter lengths to symbols
   for (uint32_t s = lane; s < 256; s += kWave)
     L.code.len[s] = 0;
@@ -999,8 +949,9 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
   const int lane = lane_id();
 
   // ---- parse + literal compaction + histogram ----
-  for (int i = lane; i < (1 << kHashLog); i += kWave)
-    L.u.table[i] = 0;
+  static_assert(sizeof(L.u.table) == 512 * sizeof(uint4), "hash table clear");
+  for (int i = lane; i < 512; i += kWave)
+    ((uint4*)L.u.table)[i] = make_uint4(0, 0, 0, 0);
   for (int i = lane; i < 256; i += kWave)
     L.hist[i] = 0;
   __syncthreads();
@@ -1160,14 +1111,7 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
     }
   } else if (nlits >= 256 && used > 1) {
     // used symbols in ascending count order: bitonic sort of (count, symbol)
-    uint32_t* keys = L.u.huf.w;
-    for (int s = lane; s < 256; s += kWave)
-      keys[s] = (L.hist[s] << 8) | (uint32_t)s;
-    __syncthreads();
-    wave_sort256(keys);
-    for (uint32_t i = lane; i < used; i += kWave)
-      L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
-    __syncthreads();
+    huf_sort_wave(L, used);
     huf_build_wave(L, used);
     {
       const uint32_t ts = huf_write_tree_wave(L);
@@ -1308,14 +1252,7 @@ __global__ __launch_bounds__(256) void zstd_table_k(const uint8_t* __restrict__ 
   const uint32_t used = wave_sum(used_local);
   uint32_t tree_size = 0;
   if (used > 1 && chunk_bytes >= 4 * kCarrier) {
-    uint32_t* keys = L.u.huf.w;
-    for (int s = lane; s < 256; s += kWave)
-      keys[s] = (L.hist[s] << 8) | (uint32_t)s;
-    wave_sync();
-    wave_sort256(keys);
-    for (uint32_t i = lane; i < used; i += kWave)
-      L.u.huf.sym[i] = (uint16_t)(keys[256 - used + i] & 0xFF);
-    wave_sync();
+    huf_sort_wave(L, used);
     huf_build_wave(L, used);
     {
       const uint32_t ts = huf_write_tree_wave(L);
