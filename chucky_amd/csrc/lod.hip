@@ -310,22 +310,14 @@ __device__ __forceinline__ T reduce_fixed(const T* x)
   }
 }
 
+// N bytes between pointers aligned to min(N, 16) (the callers check), as
+// memcpy so that register arrays of any element type may take part (no
+// type punning); it compiles to 16-byte or narrower vector moves.
 template<int N>
 __device__ __forceinline__ void copy_bytes(void* __restrict__ dst, const void* __restrict__ src)
 {
-  if constexpr (N % 16 == 0) {
-#pragma unroll
-    for (int i = 0; i < N / 16; ++i)
-      ((uint4*)dst)[i] = ((const uint4*)src)[i];
-  } else if constexpr (N == 8) {
-    *(uint64_t*)dst = *(const uint64_t*)src;
-  } else if constexpr (N == 4) {
-    *(uint32_t*)dst = *(const uint32_t*)src;
-  } else if constexpr (N == 2) {
-    *(uint16_t*)dst = *(const uint16_t*)src;
-  } else {
-    *(uint8_t*)dst = *(const uint8_t*)src;
-  }
+  constexpr int A = N < 16 ? N : 16;
+  __builtin_memcpy(__builtin_assume_aligned(dst, A), __builtin_assume_aligned(src, A), N);
 }
 
 constexpr uint32_t outputs_per_lane(int W, size_t sz)
@@ -361,7 +353,8 @@ __global__ __launch_bounds__(256) void reduce_k(T* __restrict__ values, uint64_t
         const T* v = values + src_beg + (uint64_t)b * src_n + s;
         T* o = values + dst_beg + (uint64_t)b * dst_n + ib;
         if (cnt == P && ((uintptr_t)v & 15) == 0 && ((uintptr_t)o & (min(NOUT, 16) - 1)) == 0) {
-          T x[W * P], y[P];
+          alignas(16) T x[W * P];
+          alignas(16) T y[P];
           copy_bytes<NIN>(x, v);
 #pragma unroll
           for (uint32_t p = 0; p < P; ++p)
@@ -390,6 +383,159 @@ __global__ __launch_bounds__(256) void reduce_k(T* __restrict__ values, uint64_t
     for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y)
       values[dst_beg + (uint64_t)b * dst_n + i] =
           reduce_run<T, M>(values + src_beg + (uint64_t)b * src_n + s, len);
+  }
+}
+
+// Several level pairs l -> l+1 in one launch. Windows partition every
+// level, so the top-level outputs [k0, k1) of a workgroup own one span per
+// level below them, disjoint from every other workgroup's. A full workgroup
+// whose windows all have W cells (the interior; at most W each, so W^F cells
+// for each top output means all) is regular: every lane loads 64 contiguous
+// bytes of level 0 into registers and reduces them through as many levels as
+// they cover, writing each level's values straight out; the values left per
+// lane go to LDS for the levels above. Intermediate levels are never read
+// back from memory. A ragged workgroup (edges, the last one of a batch)
+// reduces level by level through the window ends, in memory. Each window
+// reduces as in reduce_k.
+constexpr int kPyrMax = 6; // level pairs one launch fuses at most
+
+template<typename T>
+constexpr uint32_t pyr_cells() { return 64 / sizeof(T); } // level-0 cells per lane
+
+template<typename T>
+constexpr uint32_t pyr_span() { return 256 * pyr_cells<T>(); } // level-0 cells per workgroup
+
+// values a lane holds after reducing its cells through every level they cover
+template<typename T, int W>
+constexpr uint32_t pyr_rest()
+{
+  uint32_t c = pyr_cells<T>();
+  while (c >= (uint32_t)W)
+    c /= (uint32_t)W;
+  return c;
+}
+
+struct PyrArgs
+{
+  uint64_t beg[kPyrMax + 1]; // element offset of each level's values
+  uint64_t ends[kPyrMax];    // offset of the l -> l+1 window ends
+  uint32_t n[kPyrMax + 1];   // elements per batch of each level
+  uint32_t levels;           // level pairs fused (F)
+  uint32_t group;            // top-level outputs per workgroup: span / W^F
+};
+
+// n values of x to dst (16-byte pieces, or narrower, when aligned).
+template<typename T, uint32_t N>
+__device__ __forceinline__ void store_run(T* dst, const T* x)
+{
+  constexpr uint32_t B = N * sizeof(T);
+  constexpr uint32_t A = B < 16 ? B : 16;
+  if (((uintptr_t)dst & (A - 1)) == 0) {
+    copy_bytes<B>(dst, x);
+  } else {
+#pragma unroll
+    for (uint32_t i = 0; i < N; ++i)
+      dst[i] = x[i];
+  }
+}
+
+template<typename T, int M, int W>
+__global__ __launch_bounds__(256) void pyramid_k(T* __restrict__ values, const uint64_t* __restrict__ ends,
+                                                 PyrArgs a, uint32_t nb)
+{
+  constexpr uint32_t C = pyr_cells<T>(), SPAN = pyr_span<T>();
+  __shared__ alignas(16) T buf0[256 * W];
+  __shared__ alignas(16) T buf1[256 * W / 2];
+  __shared__ uint32_t lo[kPyrMax + 1], hi[kPyrMax + 1];
+  const uint32_t F = a.levels;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    uint32_t l0 = blockIdx.x * a.group, h0 = min(l0 + a.group, a.n[F]);
+    lo[F] = l0;
+    hi[F] = h0;
+    for (int l = (int)F - 1; l >= 0; --l) {
+      const uint64_t* e = ends + a.ends[l];
+      l0 = l0 ? (uint32_t)e[l0 - 1] : 0u;
+      h0 = (uint32_t)e[h0 - 1];
+      lo[l] = l0;
+      hi[l] = h0;
+    }
+  }
+  __syncthreads();
+  const bool regular = hi[0] - lo[0] == SPAN && hi[F] - lo[F] == a.group;
+  for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
+    if (regular) {
+      const T* src = values + a.beg[0] + (uint64_t)b * a.n[0] + lo[0] + tid * C;
+      alignas(16) T x[C];
+      if (((uintptr_t)src & 15) == 0) {
+        copy_bytes<C * sizeof(T)>(x, src);
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < C; ++i)
+          x[i] = src[i];
+      }
+      // levels within the lane's cells, in place: x[0, cnt) holds the
+      // current level
+      uint32_t done = 0;
+#pragma unroll
+      for (uint32_t l = 1, cnt = C; l <= kPyrMax; ++l) {
+        if (cnt < (uint32_t)W || l > F)
+          break;
+        cnt /= (uint32_t)W;
+#pragma unroll
+        for (uint32_t i = 0; i < cnt; ++i)
+          x[i] = reduce_fixed<T, M, W>(x + i * W);
+        T* dst = values + a.beg[l] + (uint64_t)b * a.n[l] + lo[l] + tid * cnt;
+        switch (cnt) {
+          case 1: store_run<T, 1>(dst, x); break;
+          case 2: store_run<T, 2>(dst, x); break;
+          case 4: store_run<T, 4>(dst, x); break;
+          case 8: store_run<T, 8>(dst, x); break;
+          case 16: store_run<T, 16>(dst, x); break;
+          default: store_run<T, 32>(dst, x); break;
+        }
+        done = l;
+      }
+      if (done == F)
+        continue;
+      // the levels above (the lane's cells are used up: R values remain):
+      // the lanes' values in LDS, W of them per output
+      constexpr uint32_t R = pyr_rest<T, W>();
+#pragma unroll
+      for (uint32_t i = 0; i < R; ++i)
+        buf0[tid * R + i] = x[i];
+      uint32_t n_in = 256 * R;
+      __syncthreads();
+      const T* in = buf0;
+      T* out = buf1;
+      for (uint32_t l = done + 1; l <= F; ++l) {
+        const uint32_t n_out = n_in / (uint32_t)W;
+        T* dst = values + a.beg[l] + (uint64_t)b * a.n[l] + lo[l];
+        for (uint32_t i = tid; i < n_out; i += 256) {
+          const T y = reduce_fixed<T, M, W>(in + i * W);
+          out[i] = y;
+          dst[i] = y;
+        }
+        __syncthreads();
+        const T* t = in;
+        in = out;
+        out = const_cast<T*>(t);
+        n_in = n_out;
+      }
+    } else {
+      for (uint32_t l = 1; l <= F; ++l) {
+        const T* src = values + a.beg[l - 1] + (uint64_t)b * a.n[l - 1];
+        T* dst = values + a.beg[l] + (uint64_t)b * a.n[l];
+        const uint64_t* e = ends + a.ends[l - 1];
+        for (uint32_t i = lo[l] + tid; i < hi[l]; i += 256) {
+          const uint64_t s0 = i ? e[i - 1] : 0;
+          dst[i] = reduce_run<T, M>(src + s0, (uint32_t)(e[i] - s0));
+        }
+        // the next level reads what this workgroup just wrote
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -564,6 +710,80 @@ hipError_t reduce_launch(int ndim, int method, T* values, uint64_t sb, uint64_t 
   }
 }
 
+// Every level pair l -> l+1 with l in [l0, l1) by pyramid_k launches of up to
+// kPyrMax pairs (W^F <= span / 4, so a workgroup owns at least 4 top
+// outputs); for more than 3 LOD dims (no fixed window) the per-level reduce.
+// Sets *rin / *rout to the bytes the launches read and write.
+template<typename T, int W>
+hipError_t pyramid_launch_w(const Plan& p, int method, T* values, const uint64_t* d_ends, int l0, int l1,
+                            hipStream_t st, hipEvent_t start, hipEvent_t stop, double* rin, double* rout)
+{
+  const uint64_t nb = p.fixed_count;
+  int l = l0;
+  while (l < l1) {
+    PyrArgs a = {};
+    uint32_t F = 0, wf = 1;
+    while (l + (int)F < l1 && (int)F < kPyrMax && wf * W <= pyr_span<T>() / 4) {
+      ++F;
+      wf *= W;
+    }
+    a.levels = F;
+    a.group = pyr_span<T>() / wf;
+    for (uint32_t k = 0; k <= F; ++k) {
+      a.beg[k] = p.level_span_begin(l + (int)k);
+      a.n[k] = (uint32_t)p.levels[l + k].lod_nelem;
+      if (k < F)
+        a.ends[k] = p.segment_begin(l + (int)k);
+    }
+    *rin += (double)nb * a.n[0] * sizeof(T);
+    for (uint32_t k = 1; k <= F; ++k)
+      *rout += (double)nb * a.n[k] * sizeof(T);
+    const dim3 grid((a.n[F] + a.group - 1) / a.group, batch_grid(nb));
+    const bool first = l == l0, last = l + (int)F == l1;
+    switch (method) {
+#define R(M)                                                                                   \
+  case M:                                                                                      \
+    launch(pyramid_k<T, M, W>, grid, dim3(256), st, first ? start : nullptr, last ? stop : nullptr, \
+           values, d_ends, a, (uint32_t)nb);                                                   \
+    break;
+      R(lod_reduce_mean)
+      R(lod_reduce_min)
+      R(lod_reduce_max)
+      R(lod_reduce_median)
+      R(lod_reduce_max_suppressed)
+      R(lod_reduce_min_suppressed)
+#undef R
+      default: return hipErrorInvalidValue;
+    }
+    if (hipGetLastError() != hipSuccess)
+      return hipErrorLaunchFailure;
+    l += (int)F;
+  }
+  return hipSuccess;
+}
+
+template<typename T>
+hipError_t pyramid_launch(const Plan& p, int method, T* values, const uint64_t* d_ends, int l0, int l1,
+                          hipStream_t st, hipEvent_t start, hipEvent_t stop, double* rin, double* rout)
+{
+  switch (p.lod_ndim) {
+    case 1: return pyramid_launch_w<T, 2>(p, method, values, d_ends, l0, l1, st, start, stop, rin, rout);
+    case 2: return pyramid_launch_w<T, 4>(p, method, values, d_ends, l0, l1, st, start, stop, rin, rout);
+    case 3: return pyramid_launch_w<T, 8>(p, method, values, d_ends, l0, l1, st, start, stop, rin, rout);
+    default: break;
+  }
+  for (int l = l0; l < l1; ++l) {
+    const uint64_t sn = p.levels[l].lod_nelem, dn = p.levels[l + 1].lod_nelem;
+    if (reduce_launch<T>(p.lod_ndim, method, values, p.level_span_begin(l), p.level_span_begin(l + 1),
+                         d_ends + p.segment_begin(l), sn, dn, p.fixed_count, st, l == l0 ? start : nullptr,
+                         l + 1 == l1 ? stop : nullptr) != hipSuccess)
+      return hipErrorLaunchFailure;
+    *rin += (double)p.fixed_count * sn * sizeof(T);
+    *rout += (double)p.fixed_count * dn * sizeof(T);
+  }
+  return hipSuccess;
+}
+
 // One epoch of the LOD path for element type T.
 template<typename T>
 int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
@@ -590,21 +810,10 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
       last_reduce = l;
   if (last_reduce >= 0) {
     double rin = 0, rout = 0;
-    if (timing_pair(&ta, &tb))
+    if (timing_pair(&ta, &tb) ||
+        pyramid_launch<T>(p, p.config.reduce_method, lod, s->d_ends, 0, last_reduce + 1, st, ta, tb, &rin,
+                          &rout) != hipSuccess)
       return 1;
-    bool first = true;
-    for (int l = 0; l + 1 < p.nlod; ++l) {
-      const uint64_t sn = p.levels[l].lod_nelem, dn = p.levels[l + 1].lod_nelem;
-      if (!dn)
-        continue;
-      if (reduce_launch<T>(p.lod_ndim, p.config.reduce_method, lod, p.level_span_begin(l),
-                           p.level_span_begin(l + 1), s->d_ends + p.segment_begin(l), sn, dn,
-                           p.fixed_count, st, first ? ta : nullptr, l == last_reduce ? tb : nullptr) != hipSuccess)
-        return 1;
-      first = false;
-      rin += nb * sn * bpe;
-      rout += nb * dn * bpe;
-    }
     timing_add(s, ta, tb, &stream_metrics::lod_reduce, rin, rout);
   }
   uint32_t mask = p.append_downsample ? 1u : (uint32_t)((1ull << p.nlod) - 1);
