@@ -58,6 +58,10 @@ constexpr uint64_t kWorkerScratch = (kLitsBytes + kSeqBytes + kSeqOutBytes + kAu
 constexpr int kSymPerLane = 8; // Huffman symbols per lane per round
 // one round: <= 64 lanes * 8 symbols * 11 bits, plus < 8 carried bits
 constexpr int kStageWords = (kWave * kSymPerLane * zcore::kHufMaxBits + 8 + 31) / 32 + 4;
+// The literal streams stage up to 4 rounds before copying complete bytes out.
+constexpr int kRoundBits = kWave * kSymPerLane * zcore::kHufMaxBits;
+constexpr int kStreamWords = (4 * kRoundBits + 8 + 31) / 32 + 4;
+constexpr uint32_t kStreamFlushBits = (uint32_t)(kStreamWords - 4) * 32 - (uint32_t)kRoundBits;
 
 // ~10.2 KiB per wavefront, so that 15 wavefronts fit a CU's LDS. The union
 // is the parse's hash table, then the Huffman builder's scratch, then the
@@ -84,7 +88,7 @@ struct Lds
     struct
     {
       uint32_t cw[256]; // code value | length << 16
-      uint32_t stage[kStageWords];
+      uint32_t stage[kStreamWords]; // >= kStageWords
     } enc;
   } u;
   uint32_t hist[256];
@@ -198,15 +202,18 @@ __device__ __forceinline__ uint64_t round_syms(const uint8_t* lits, long a, long
 
 // Encodes lits[a, b) as one backward Huffman stream at out; returns its size.
 // Each round the wave takes 512 symbols, highest first: every lane packs its
-// 8 codes into a 128-bit word, a wave prefix sum of the bit counts places the
-// words, LDS atomic-ORs assemble them in `stage`, and complete bytes are
-// copied out. The next round's symbols are loaded before this round's codes
-// are looked up, so the global load latency overlaps the LDS work.
+// 8 codes (at most 88 bits) into two 64-bit accumulators, a wave prefix sum
+// of the bit counts places the lanes' words, funnel shifts align them and
+// LDS atomic-ORs assemble them in `stage`; complete bytes are copied out
+// once the stage could not take another round. The next round's symbols are
+// loaded before this round's codes are looked up, so the load latency
+// overlaps the LDS work.
 __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t a, uint32_t b,
                                     const uint32_t* cw, uint32_t* stage)
 {
+  static_assert(5 * zcore::kHufMaxBits <= 64, "five codes fit one accumulator");
   const int lane = lane_id();
-  for (int i = lane; i < kStageWords; i += kWave)
+  for (int i = lane; i < kStreamWords; i += kWave)
     stage[i] = 0;
   __syncthreads();
   uint32_t flushed = 0; // bytes already copied out
@@ -218,32 +225,48 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
     const long next = top - (long)kWave * kSymPerLane;
     int nskip = kSymPerLane;
     const uint64_t nq = next >= (long)a ? round_syms(lits, a, next, lane, &nskip) : 0;
-    unsigned __int128 bits = 0;
-    uint32_t nb = 0;
+    // symbols 7..3 into A, 2..0 into B (a skipped symbol adds no bits)
+    uint64_t A = 0, B = 0;
+    uint32_t na = 0, nbb = 0;
 #pragma unroll
-    for (int j = kSymPerLane - 1; j >= 0; --j) {
-      if (j >= skip) {
-        const uint32_t c = cw[(uint32_t)(q >> (8 * j)) & 0xFF];
-        bits |= (unsigned __int128)(c & 0xFFFF) << nb;
-        nb += c >> 16;
-      }
+    for (int j = kSymPerLane - 1; j >= 3; --j) {
+      const uint32_t c = j >= skip ? cw[(uint32_t)(q >> (8 * j)) & 0xFF] : 0u;
+      A |= (uint64_t)(c & 0xFFFF) << na;
+      na += c >> 16;
     }
+#pragma unroll
+    for (int j = 2; j >= 0; --j) {
+      const uint32_t c = j >= skip ? cw[(uint32_t)(q >> (8 * j)) & 0xFF] : 0u;
+      B |= (uint64_t)(c & 0xFFFF) << nbb;
+      nbb += c >> 16;
+    }
+    const uint32_t nb = na + nbb;
+    // the lane's bits: lo | hi << 64, hi below 2^24
+    const uint64_t lo = A | (B << na);
+    const uint32_t hi = na ? (uint32_t)(B >> (64 - na)) : 0u;
     const uint32_t incl = wave_incl_scan(nb);
     const uint32_t total = wave_bcast(incl, kWave - 1);
     if (nb) {
       const uint32_t p = rel + incl - nb;
-      const unsigned __int128 sh = bits << (p & 31);
+      const uint32_t sh = p & 31;
+      const uint32_t v0 = (uint32_t)lo, v1 = (uint32_t)(lo >> 32);
+      const uint32_t w0 = v0 << sh;
+      const uint32_t w1 = sh ? __builtin_amdgcn_alignbit(v1, v0, 32 - sh) : v1;
+      const uint32_t w2 = sh ? __builtin_amdgcn_alignbit(hi, v1, 32 - sh) : hi;
+      const uint32_t w3 = sh ? hi >> (32 - sh) : 0u;
       uint32_t* w = stage + (p >> 5);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t v = (uint32_t)(sh >> (32 * k));
-        if (v)
-          atomicOr(&w[k], v);
-      }
+      if (w0)
+        atomicOr(&w[0], w0);
+      if (w1)
+        atomicOr(&w[1], w1);
+      if (w2)
+        atomicOr(&w[2], w2);
+      if (w3)
+        atomicOr(&w[3], w3);
     }
-    __syncthreads();
     rel += total;
     if (next < (long)a) {
+      __syncthreads();
       if (lane == 0)
         stage[rel >> 5] |= 1u << (rel & 31); // end mark
       __syncthreads();
@@ -252,16 +275,19 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
       __syncthreads();
       return flushed + nbytes;
     }
-    const uint32_t full = rel >> 3;
-    wave_copy(out + flushed, (const uint8_t*)stage, full);
-    const uint32_t carry = ((const uint8_t*)stage)[full];
-    __syncthreads();
-    const uint32_t used_words = (rel + 31) >> 5;
-    for (uint32_t i = lane; i < used_words; i += kWave)
-      stage[i] = (i == 0) ? (carry & ((1u << (rel & 7)) - 1u)) : 0u;
-    __syncthreads();
-    flushed += full;
-    rel &= 7;
+    if (rel > kStreamFlushBits) {
+      __syncthreads();
+      const uint32_t full = rel >> 3;
+      wave_copy(out + flushed, (const uint8_t*)stage, full);
+      const uint32_t carry = ((const uint8_t*)stage)[full];
+      __syncthreads();
+      const uint32_t used_words = (rel + 31) >> 5;
+      for (uint32_t i = lane; i < used_words; i += kWave)
+        stage[i] = (i == 0) ? (carry & ((1u << (rel & 7)) - 1u)) : 0u;
+      __syncthreads();
+      flushed += full;
+      rel &= 7;
+    }
     top = next;
     q = nq;
     skip = nskip;
