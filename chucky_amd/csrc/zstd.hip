@@ -728,55 +728,85 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
 }
 
 // The block's literals (every byte outside the matches, in order) into lits,
-// with their byte histogram: the matches mark their bytes in a bitmap (aux,
-// one bit per block byte), then each round the wave compacts 256 input bytes
-// by a prefix sum of the unmarked counts.
+// with their byte histogram. Per 64 KiB of the block: the matches mark their
+// bytes in an LDS bitmap (the parse's hash table space, free by now), then
+// each round every lane takes 16 input bytes and their 16 bitmap bits, a
+// wave prefix sum of the unmarked counts places them, and the next round's
+// bytes are loaded before this round's are stored.
 __device__ void gather_literals(const uint8_t* in, uint32_t len, const Seq* seqs, uint32_t nseq,
                                 uint32_t* bitmap, uint8_t* lits, uint32_t* hist)
 {
-  const int lane = lane_id();
-  const uint32_t words = (len + 31) / 32;
-  for (uint32_t w = lane; w < words; w += kWave)
-    bitmap[w] = 0;
+  constexpr uint32_t kPart = 65536; // bytes one bitmap covers
+  static_assert(kPart / 8 <= sizeof(uint16_t) << kHashLog, "bitmap fits the hash table space");
+  const uint32_t lane = (uint32_t)lane_id();
   __threadfence_block();
-  wave_sync();
-  for (uint32_t i = lane; i < nseq; i += kWave) {
-    const Seq s = seqs[i];
-    uint32_t a = s.pos;
-    const uint32_t e = s.pos + s.ml;
-    while (a < e) {
-      const uint32_t w = a >> 5, bit = a & 31;
-      const uint32_t n = min(32u - bit, e - a);
-      const uint32_t m = (n == 32 ? ~0u : ((1u << n) - 1u)) << bit;
-      if (m == ~0u)
-        bitmap[w] = m; // a whole word of one match is this lane's alone
-      else
-        atomicOr(&bitmap[w], m);
-      a += n;
-    }
-  }
-  __threadfence_block();
-  wave_sync();
   uint32_t out = 0;
-  for (uint32_t base = 0; base < len; base += 4 * kWave) {
-    const uint32_t p = base + 4u * (uint32_t)lane;
-    uint32_t lit = 0, v = 0;
-    if (p < len) {
-      v = ld32u(in + p);
-      const uint32_t valid = len - p >= 4 ? 0xFu : ((1u << (len - p)) - 1u);
-      lit = ~(bitmap[p >> 5] >> (p & 31)) & valid;
-    }
-    const uint32_t cnt = (uint32_t)__popc(lit);
-    const uint32_t incl = wave_incl_scan(cnt);
-    uint32_t d = out + incl - cnt;
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (lit & (1u << j)) {
-        const uint32_t byte = (v >> (8 * j)) & 0xFF;
-        lits[d++] = (uint8_t)byte;
-        atomicAdd(&hist[byte], 1u);
+  for (uint32_t base = 0; base < len; base += kPart) {
+    const uint32_t n = min(len - base, kPart);
+    const uint32_t words = (n + 31) / 32;
+    for (uint32_t w = lane; w < words; w += kWave)
+      bitmap[w] = 0;
+    wave_sync();
+    for (uint32_t i = lane; i < nseq; i += kWave) {
+      const Seq sq = seqs[i];
+      uint32_t lo = sq.pos, e = sq.pos + sq.ml;
+      if (e <= base || lo >= base + n)
+        continue;
+      lo = lo > base ? lo - base : 0u;
+      e = min(e - base, n);
+      while (lo < e) {
+        const uint32_t w = lo >> 5, bit = lo & 31;
+        const uint32_t c = min(32u - bit, e - lo);
+        const uint32_t m = (c == 32 ? ~0u : ((1u << c) - 1u)) << bit;
+        atomicOr(&bitmap[w], m);
+        lo += c;
       }
-    out += wave_bcast(incl, kWave - 1);
+    }
+    wave_sync();
+    const uint8_t* src = in + base;
+    // round r: lane bytes [16 (64 r + lane), +16); inputs carry 64 bytes of
+    // slack, so a partial last piece reads past n harmlessly
+    uint32_t p = 16u * lane;
+    uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+    if (p < n) {
+      d0 = ld32u(src + p);
+      d1 = ld32u(src + p + 4);
+      d2 = ld32u(src + p + 8);
+      d3 = ld32u(src + p + 12);
+    }
+    for (uint32_t r0 = 0; r0 < n; r0 += 16u * kWave) {
+      const uint32_t q = p + 16u * kWave;
+      uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+      if (q < n) {
+        e0 = ld32u(src + q);
+        e1 = ld32u(src + q + 4);
+        e2 = ld32u(src + q + 8);
+        e3 = ld32u(src + q + 12);
+      }
+      uint32_t lit = 0;
+      if (p < n) {
+        const uint32_t valid = n - p >= 16 ? 0xFFFFu : ((1u << (n - p)) - 1u);
+        lit = ~(bitmap[p >> 5] >> (p & 16)) & valid;
+      }
+      const uint32_t cnt = (uint32_t)__popc(lit);
+      const uint32_t incl = wave_incl_scan(cnt);
+      uint32_t dst = out + incl - cnt;
+      const uint32_t dw[4] = { d0, d1, d2, d3 };
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (lit & (1u << j)) {
+          const uint32_t byte = (dw[j >> 2] >> (8 * (j & 3))) & 0xFF;
+          lits[dst++] = (uint8_t)byte;
+          atomicAdd(&hist[byte], 1u);
+        }
+      out += wave_bcast(incl, kWave - 1);
+      p = q;
+      d0 = e0;
+      d1 = e1;
+      d2 = e2;
+      d3 = e3;
+    }
+    wave_sync();
   }
   __threadfence_block();
   wave_sync();
@@ -1092,7 +1122,7 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
     count_bytes(in, len, L.hist);
     lits = const_cast<uint8_t*>(in);
   } else {
-    gather_literals(in, len, seqs, nseq, aux, lits, L.hist);
+    gather_literals(in, len, seqs, nseq, (uint32_t*)L.u.table, lits, L.hist);
   }
   nlits += len - anchor;
   __syncthreads();
