@@ -157,7 +157,7 @@ def run(name, frames, steps, warmup, pipelines, host_input, options=None):
 
 LOD_PASSES = {"lod_gather": "tiles_k (linear epoch -> level-0 Morton)",
               "lod_reduce": "pyramid_k (every level from level 0, one launch)",
-              "lod_append_fold": "fold_k + emit_k (append-dim fold)",
+              "lod_append_fold": "fold_emit_k (append-dim fold and emit, all levels, one launch)",
               "lod_morton_chunk": "tiles_k (Morton levels -> chunk pool, all levels)"}
 
 

@@ -33,7 +33,7 @@ __device__ __forceinline__ uint64_t item_bytes(const SegmentSource& s, uint32_t 
 __global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
   SegmentSource src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
   uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* chunk_sizes,
-  uint64_t* offsets)
+  uint64_t* offsets, uint64_t* host_sizes, uint64_t* host_offsets)
 {
   __shared__ uint64_t part[kPlanThreads];
   const int t = threadIdx.x;
@@ -74,10 +74,17 @@ __global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
   uint64_t run = part[t] - local;
   for (uint64_t i = b; i < e; ++i) {
     offsets[i] = run;
+    if (host_offsets)
+      host_offsets[i] = run;
+    if (host_sizes)
+      host_sizes[i] = chunk_sizes[i];
     run += value(i);
   }
-  if (t == kPlanThreads - 1)
+  if (t == kPlanThreads - 1) {
     offsets[n_slots] = part[t];
+    if (host_offsets)
+      host_offsets[n_slots] = part[t];
+  }
 }
 
 __device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p)
@@ -174,10 +181,11 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
 hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gather,
                                  const uint32_t* perm, uint64_t n_items, uint64_t n_slots,
                                  uint64_t group, uint64_t page_size, uint64_t* chunk_sizes,
-                                 uint64_t* offsets, hipStream_t stream)
+                                 uint64_t* offsets, hipStream_t stream, uint64_t* host_sizes,
+                                 uint64_t* host_offsets)
 {
   hipLaunchKernelGGL(aggregate_plan_k, dim3(1), dim3(kPlanThreads), 0, stream, src, gather, perm,
-                     n_items, n_slots, group, page_size, chunk_sizes, offsets);
+                     n_items, n_slots, group, page_size, chunk_sizes, offsets, host_sizes, host_offsets);
   return hipGetLastError();
 }
 

@@ -41,11 +41,14 @@ struct SegmentSource
 
 // One workgroup: chunk_sizes[perm[i]] = size(gather[i]) for i < n_items,
 // zero elsewhere; optional page padding on the last slot of each group of
-// `group` slots; offsets = exclusive scan (n_slots + 1 entries).
+// `group` slots; offsets = exclusive scan (n_slots + 1 entries). When
+// host_sizes / host_offsets (pinned host memory) are given, the kernel also
+// writes both arrays there, so no copy has to follow it.
 hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gather,
                                  const uint32_t* perm, uint64_t n_items, uint64_t n_slots,
                                  uint64_t group, uint64_t page_size, uint64_t* chunk_sizes,
-                                 uint64_t* offsets, hipStream_t stream);
+                                 uint64_t* offsets, hipStream_t stream, uint64_t* host_sizes = nullptr,
+                                 uint64_t* host_offsets = nullptr);
 
 // Copy every item's segments to dst + offsets[perm[i]] (dst may be device or
 // pinned host memory).

@@ -93,10 +93,10 @@ __device__ __forceinline__ int64_t row_offset(uint32_t r, const TileGeom& g)
 // rows are padded by one piece, which spreads the Z-order accesses of a wave
 // over the banks; the padded index is linear over disjoint bits as well.
 template<typename U, int D, bool ToMorton, bool VecArray>
-__global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __restrict__ array,
-                                               const uint32_t* __restrict__ tiles,
-                                               const uint64_t* __restrict__ fixed, uint64_t n,
-                                               uint32_t nb, TileGeom g)
+__device__ __forceinline__ void tiles_body(U* __restrict__ morton, U* __restrict__ array,
+                                           const uint32_t* __restrict__ tiles,
+                                           const uint64_t* __restrict__ fixed, uint64_t n, uint32_t nb,
+                                           const TileGeom& g, uint32_t tile)
 {
   constexpr int S = lod_tile_side_log(D, sizeof(U));
   constexpr uint32_t cells = 1u << (S * D);
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
 #pragma unroll
   for (uint32_t j = 0; j < V; ++j)
     jr[j] = at(z_to_row<D, S>(j, g));
-  const uint32_t base = tiles[2 * blockIdx.x], origin = tiles[2 * blockIdx.x + 1];
+  const uint32_t base = tiles[2 * tile], origin = tiles[2 * tile + 1];
   for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
     U* m = morton + (uint64_t)b * n + base;
     U* a = array + fixed[b] + origin;
@@ -197,6 +197,42 @@ __global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __rest
     }
     __syncthreads();
   }
+}
+
+template<typename U, int D, bool ToMorton, bool VecArray>
+__global__ __launch_bounds__(256) void tiles_k(U* __restrict__ morton, U* __restrict__ array,
+                                               const uint32_t* __restrict__ tiles,
+                                               const uint64_t* __restrict__ fixed, uint64_t n,
+                                               uint32_t nb, TileGeom g)
+{
+  tiles_body<U, D, ToMorton, VecArray>(morton, array, tiles, fixed, n, nb, g, blockIdx.x);
+}
+
+// The Morton -> chunk-pool scatter of several levels in one launch (levels
+// whose tiles share one geometry): workgroup x takes tile x - first[k] of
+// level k, the level whose range [first[k], first[k + 1]) holds x.
+constexpr int kMoveMax = LOD_MAX_LEVELS;
+
+template<typename U>
+struct MoveLevels
+{
+  U* morton[kMoveMax];
+  const uint32_t* tiles[kMoveMax];
+  const uint64_t* fixed[kMoveMax];
+  uint64_t n[kMoveMax];
+  uint32_t first[kMoveMax + 1];
+  int nlev;
+};
+
+template<typename U, int D, bool VecArray>
+__global__ __launch_bounds__(256) void tiles_levels_k(MoveLevels<U> m, U* __restrict__ array, uint32_t nb,
+                                                      TileGeom g)
+{
+  int k = 0;
+  while (k + 1 < m.nlev && blockIdx.x >= m.first[k + 1])
+    ++k;
+  tiles_body<U, D, false, VecArray>(m.morton[k], array, m.tiles[k], m.fixed[k], m.n[k], nb, g,
+                                    blockIdx.x - m.first[k]);
 }
 
 // Cells outside full tiles: one (rank, offset) pair per work-item.
@@ -551,29 +587,6 @@ __device__ __forceinline__ T add_shift(T a, T b, int s)
 }
 
 template<typename T>
-__global__ __launch_bounds__(256) void fold_k(T* __restrict__ accum, const T* __restrict__ x,
-                                              uint64_t n, uint32_t count, int level, int method)
-{
-  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= n)
-    return;
-  const T v = x[g];
-  if (count == 0) {
-    accum[g] = v;
-    return;
-  }
-  if (method == lod_reduce_mean)
-    accum[g] = add_shift<T>(accum[g], v, level);
-  else if (method == lod_reduce_min) {
-    if (v < accum[g])
-      accum[g] = v;
-  } else if (method == lod_reduce_max) {
-    if (v > accum[g])
-      accum[g] = v;
-  }
-}
-
-template<typename T>
 __global__ __launch_bounds__(256) void emit_k(T* __restrict__ dst, const T* __restrict__ accum,
                                               uint64_t n, uint32_t count, int method)
 {
@@ -587,6 +600,58 @@ __global__ __launch_bounds__(256) void emit_k(T* __restrict__ dst, const T* __re
     }
   }
   dst[g] = accum[g];
+}
+
+// The append-dim fold of every level >= 1 in one launch: level lv's
+// elements follow the previous levels' (end[] holds the running totals).
+// Each element folds the epoch's value into its accumulator (the first
+// epoch of a window starts it) and, when the level's window of 2^lv epochs
+// completes, also writes the emitted value over the level's Morton values:
+// the same operations as fold_k followed by emit_k.
+constexpr int kFoldMax = LOD_MAX_LEVELS;
+
+template<typename T>
+struct FoldLevels
+{
+  T* x[kFoldMax];         // the level's Morton values (the emitted value goes here)
+  T* acc[kFoldMax];       // its accumulator
+  uint64_t end[kFoldMax]; // running element totals
+  uint32_t count[kFoldMax];
+  uint32_t emit[kFoldMax]; // epochs in the window when it completes now, else 0
+  int level[kFoldMax];
+  int nlev;
+};
+
+template<typename T>
+__global__ __launch_bounds__(256) void fold_emit_k(FoldLevels<T> f, int method)
+{
+  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  int k = 0;
+  while (k < f.nlev && g >= f.end[k])
+    ++k;
+  if (k >= f.nlev)
+    return;
+  const uint64_t i = g - (k ? f.end[k - 1] : 0);
+  const T v = f.x[k][i];
+  T a = v;
+  if (f.count[k]) {
+    a = f.acc[k][i];
+    if (method == lod_reduce_mean)
+      a = add_shift<T>(a, v, f.level[k]);
+    else if (method == lod_reduce_min)
+      a = v < a ? v : a;
+    else if (method == lod_reduce_max)
+      a = v > a ? v : a;
+  }
+  f.acc[k][i] = a;
+  if (f.emit[k]) {
+    T out = a;
+    if constexpr (is_fp<T>) {
+      if (method == lod_reduce_mean)
+        out = (T)(a / (T)f.emit[k]);
+    }
+    f.x[k][i] = out;
+  }
 }
 
 // Element-wise kernels count work-items in 32 bits, which lod_init checks
@@ -665,6 +730,90 @@ hipError_t move_launch(const LodTiles& t, const uint32_t* d_tiles, const uint32_
   if (nedge)
     launch(edge_k<U, ToMorton>, dim3((nedge + 255) / 256, gy), dim3(256), st, ntiles ? nullptr : start,
            stop, morton, array, d_edge, nedge, fixed, n, (uint32_t)nb);
+  return hipGetLastError();
+}
+
+// Morton -> chunk-pool scatter of the levels in `lvs`: one tiles_levels_k
+// launch when their tiles share one geometry (the usual case: every level's
+// chunks have the same shape), then the edge cells level by level; otherwise
+// one move_launch per level.
+template<typename T>
+hipError_t scatter_levels(tile_stream_gpu* s, const std::vector<int>& lvs, T* lod, T* pool, hipStream_t st,
+                          hipEvent_t start, hipEvent_t stop)
+{
+  using U = typename Bits<sizeof(T)>::type;
+  const Plan& p = s->plan;
+  const uint64_t nb = p.fixed_count;
+  if (lvs.empty() || nb == 0)
+    return hipSuccess;
+  const LodTiles& t0 = s->luts.chunk_tiles[lvs[0]];
+  bool same = lvs.size() <= (size_t)kMoveMax && t0.ndim >= 1 && t0.ndim <= 3;
+  for (int lv : lvs) {
+    const LodTiles& t = s->luts.chunk_tiles[lv];
+    same = same && t.ndim == t0.ndim && t.side_log == t0.side_log && t.vec == t0.vec;
+    for (int q = 0; q < t0.ndim && same; ++q)
+      same = t.stride[q] == t0.stride[q];
+  }
+  if (!same) {
+    for (size_t i = 0; i < lvs.size(); ++i) {
+      const int lv = lvs[i];
+      if (move_launch<T, false>(s->luts.chunk_tiles[lv], s->d_chunk_tiles[lv], s->d_chunk_edge[lv],
+                                lod + p.level_span_begin(lv), pool, s->d_chunk_fixed[lv], p.levels[lv].lod_nelem,
+                                nb, st, i == 0 ? start : nullptr, i + 1 == lvs.size() ? stop : nullptr) !=
+          hipSuccess)
+        return hipErrorLaunchFailure;
+    }
+    return hipSuccess;
+  }
+  MoveLevels<U> m = {};
+  uint32_t tiles = 0;
+  int last_edge = -1;
+  for (size_t i = 0; i < lvs.size(); ++i) {
+    const int lv = lvs[i];
+    const LodTiles& t = s->luts.chunk_tiles[lv];
+    m.morton[i] = (U*)(lod + p.level_span_begin(lv));
+    m.tiles[i] = s->d_chunk_tiles[lv];
+    m.fixed[i] = s->d_chunk_fixed[lv];
+    m.n[i] = p.levels[lv].lod_nelem;
+    m.first[i] = tiles;
+    tiles += (uint32_t)(t.tiles.size() / 2);
+    if (!t.edge.empty())
+      last_edge = (int)i;
+  }
+  m.first[lvs.size()] = tiles;
+  m.nlev = (int)lvs.size();
+  const uint32_t gy = batch_grid(nb);
+  if (tiles) {
+    const TileGeom g = geom_of(t0);
+    const bool vec = t0.vec && ((uintptr_t)pool & 15) == 0;
+    hipEvent_t e1 = last_edge < 0 ? stop : nullptr;
+    switch (t0.ndim) {
+#define K(D)                                                                                     \
+  case D:                                                                                        \
+    if (vec)                                                                                     \
+      launch(tiles_levels_k<U, D, true>, dim3(tiles, gy), dim3(256), st, start, e1, m, (U*)pool, \
+             (uint32_t)nb, g);                                                                   \
+    else                                                                                         \
+      launch(tiles_levels_k<U, D, false>, dim3(tiles, gy), dim3(256), st, start, e1, m,          \
+             (U*)pool, (uint32_t)nb, g);                                                         \
+    break;
+      K(1)
+      K(2)
+      K(3)
+#undef K
+      default: return hipErrorInvalidValue;
+    }
+  }
+  for (size_t i = 0; i < lvs.size(); ++i) {
+    const int lv = lvs[i];
+    const LodTiles& t = s->luts.chunk_tiles[lv];
+    const uint32_t nedge = (uint32_t)(t.edge.size() / 2);
+    if (!nedge)
+      continue;
+    launch(edge_k<U, false>, dim3((nedge + 255) / 256, gy), dim3(256), st, tiles || (int)i > 0 ? nullptr : start,
+           (int)i == last_edge ? stop : nullptr, m.morton[i], (U*)pool, s->d_chunk_edge[lv], nedge,
+           s->d_chunk_fixed[lv], m.n[i], (uint32_t)nb);
+  }
   return hipGetLastError();
 }
 
@@ -822,27 +971,31 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
     double fin = 0, fout = 0;
     if (timing_begin(s, &ta))
       return 1;
+    FoldLevels<T> f = {};
+    uint64_t tot = 0;
     for (int lv = 1; lv < p.nlod; ++lv) {
       const uint64_t n = p.fixed_count * p.levels[lv].lod_nelem;
-      hipLaunchKernelGGL(fold_k<T>, grid_of(n), dim3(256), 0, st, acc + p.append_accum_offset(lv),
-                         lod + p.level_span_begin(lv), n, s->append_counts[lv], lv,
-                         (int)p.config.append_reduce_method);
+      if (!n)
+        continue;
+      const int k = f.nlev++;
+      f.x[k] = lod + p.level_span_begin(lv);
+      f.acc[k] = acc + p.append_accum_offset(lv);
+      tot += n;
+      f.end[k] = tot;
+      f.count[k] = s->append_counts[lv];
+      f.level[k] = lv;
       fin += (s->append_counts[lv] ? 2.0 : 1.0) * n * bpe;
       fout += (double)n * bpe;
-    }
-    for (int lv = 1; lv < p.nlod; ++lv) {
       s->append_counts[lv]++;
       if (s->append_counts[lv] >= (1u << lv)) {
-        const uint64_t n = p.fixed_count * p.levels[lv].lod_nelem;
-        hipLaunchKernelGGL(emit_k<T>, grid_of(n), dim3(256), 0, st, lod + p.level_span_begin(lv),
-                           (const T*)(acc + p.append_accum_offset(lv)), n, s->append_counts[lv],
-                           (int)p.config.append_reduce_method);
-        fin += (double)n * bpe;
+        f.emit[k] = s->append_counts[lv];
         fout += (double)n * bpe;
         s->append_counts[lv] = 0;
         mask |= 1u << lv;
       }
     }
+    if (tot)
+      hipLaunchKernelGGL(fold_emit_k<T>, grid_of(tot), dim3(256), 0, st, f, (int)p.config.append_reduce_method);
     if (timing_end(s, ta, &stream_metrics::lod_append_fold, fin, fout))
       return 1;
   }
@@ -856,18 +1009,14 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
       last_lv = lv;
     }
   if (first_lv >= 0) {
-    if (timing_pair(&ta, &tb))
+    std::vector<int> lvs;
+    for (int lv = first_lv; lv <= last_lv; ++lv)
+      if ((mask & (1u << lv)) && p.levels[lv].lod_nelem) {
+        lvs.push_back(lv);
+        sbytes += nb * p.levels[lv].lod_nelem * bpe;
+      }
+    if (timing_pair(&ta, &tb) || scatter_levels<T>(s, lvs, lod, pool, st, ta, tb) != hipSuccess)
       return 1;
-    for (int lv = first_lv; lv <= last_lv; ++lv) {
-      if (!(mask & (1u << lv)) || !p.levels[lv].lod_nelem)
-        continue;
-      if (move_launch<T, false>(s->luts.chunk_tiles[lv], s->d_chunk_tiles[lv], s->d_chunk_edge[lv],
-                                lod + p.level_span_begin(lv), pool, s->d_chunk_fixed[lv],
-                                p.levels[lv].lod_nelem, p.fixed_count, st, lv == first_lv ? ta : nullptr,
-                                lv == last_lv ? tb : nullptr) != hipSuccess)
-        return 1;
-      sbytes += nb * p.levels[lv].lod_nelem * bpe;
-    }
     timing_add(s, ta, tb, &stream_metrics::lod_morton_chunk, sbytes, sbytes);
   }
   if (hipGetLastError() != hipSuccess)

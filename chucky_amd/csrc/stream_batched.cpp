@@ -192,10 +192,10 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
       continue;
     const LevelInfo& L = s->plan.levels[lv];
     const uint64_t M = L.agg.chunks_per_epoch, C = L.agg.covering_count;
+    // the plan kernel writes sizes and offsets to pinned memory as well: no
+    // copy nodes (each node costs host time at every graph launch)
     BCHECK(launch_aggregate_plan(src, P.d_gather[lv], P.d_perm[lv], M, C, L.agg.cps_inner, 0,
-                                 P.d_sizes[lv], P.d_offsets[lv], st));
-    BCHECK(hipMemcpyAsync(H.h_offsets[lv], P.d_offsets[lv], (C + 1) * 8, hipMemcpyDeviceToHost, st));
-    BCHECK(hipMemcpyAsync(H.h_sizes[lv], P.d_sizes[lv], C * 8, hipMemcpyDeviceToHost, st));
+                                 P.d_sizes[lv], P.d_offsets[lv], st, H.h_sizes[lv], H.h_offsets[lv]));
     if (b->dma)
       BCHECK(launch_aggregate_gather(src, P.d_gather[lv], P.d_perm[lv], M, P.d_offsets[lv],
                                      H.d_stage + H.level_base[lv], st, 0));
