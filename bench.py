@@ -371,6 +371,9 @@ def main():
                          "identity layout), the reference for counter-based HBM traffic")
     ap.add_argument("--batched-d2h", type=int, default=0,
                     help="batched pipeline D2H: 0 = device gather + DMA, 1 = gather kernel into host")
+    ap.add_argument("--batched-launch", type=int, default=0,
+                    help="batched pipeline per-epoch work: 0 = autotuned, 1 = captured graphs, "
+                         "2 = direct launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -388,7 +391,8 @@ def main():
     runs, data, _ = run(args.workload, args.frames, args.steps, args.warmup, order, args.host_input,
                         {"zstd_block_bytes": args.zstd_block,
                          "host_gather_workgroups": args.gather_wg,
-                         "batched_d2h": args.batched_d2h})
+                         "batched_d2h": args.batched_d2h,
+                         "batched_launch": args.batched_launch})
     head = runs["batched" if "batched" in runs else order[0]]
     value, head_sum = summarize(head, args.steps)
     m = head["metrics"]

@@ -30,8 +30,8 @@ __device__ __forceinline__ uint64_t item_bytes(const SegmentSource& s, uint32_t 
   return t;
 }
 
-__global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
-  SegmentSource src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
+__device__ __forceinline__ void plan_body(
+  const SegmentSource& src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
   uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* chunk_sizes,
   uint64_t* offsets, uint64_t* host_sizes, uint64_t* host_offsets)
 {
@@ -87,6 +87,23 @@ __global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
   }
 }
 
+__global__ __launch_bounds__(kPlanThreads) void aggregate_plan_k(
+  SegmentSource src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
+  uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* chunk_sizes,
+  uint64_t* offsets, uint64_t* host_sizes, uint64_t* host_offsets)
+{
+  plan_body(src, gather, perm, n_items, n_slots, group, page, chunk_sizes, offsets, host_sizes,
+            host_offsets);
+}
+
+// One workgroup per level.
+__global__ __launch_bounds__(kPlanThreads) void aggregate_plan_levels_k(SegmentSource src, AggLevels a)
+{
+  const int k = blockIdx.x;
+  plan_body(src, a.gather[k], a.perm[k], a.n_items[k], a.n_slots[k], a.group[k], 0, a.sizes[k],
+            a.offsets[k], a.host_sizes[k], a.host_offsets[k]);
+}
+
 __device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p)
 {
   const uintptr_t a = (uintptr_t)p;
@@ -129,6 +146,42 @@ __device__ __forceinline__ void block_copy(uint8_t* out, const uint8_t* in, uint
 // into fixed pieces) are walked as (item, piece) pairs the same way.
 // A gather into pinned host memory runs at PCIe speed, so it is given only a
 // few CUs and leaves the rest to the compressor and transpose beside it.
+// One (item, segment) pair of a level (uniform across the workgroup).
+__device__ __forceinline__ void gather_pair(const SegmentSource& src, const uint32_t* __restrict__ gather,
+                                            const uint32_t* __restrict__ perm,
+                                            const uint64_t* __restrict__ offsets, uint8_t* __restrict__ dst,
+                                            uint64_t pr, uint32_t* part)
+{
+  const uint64_t item = pr / src.nseg;
+  const uint32_t seg = (uint32_t)(pr - item * src.nseg);
+  const uint32_t chunk = gather[item];
+  if (src.seg_sizes) {
+    const uint32_t* sz = src.seg_sizes + (uint64_t)chunk * src.nseg;
+    const uint32_t nb = sz[seg];
+    if (nb == 0)
+      return;  // uniform across the workgroup: no barrier is skipped by part of it
+    // a chunk's compressed bytes (below its slot stride) fit 32 bits
+    uint32_t before = 0;
+    for (uint32_t j = threadIdx.x; j < seg; j += 256)
+      before += sz[j];
+    before = wave_sum(before);
+    if ((threadIdx.x & 63) == 0)
+      part[threadIdx.x >> 6] = before;
+    __syncthreads();
+    before = part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+    block_copy(dst + offsets[perm[item]] + before,
+               src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nb);
+    return;
+  }
+  const uint64_t seg_off = (uint64_t)seg * src.const_size;
+  const uint64_t nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
+  if (nbytes == 0 || seg_off >= src.const_total)
+    return;
+  block_copy(dst + offsets[perm[item]] + seg_off,
+             src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nbytes);
+}
+
 __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint32_t* __restrict__ gather,
                                                           const uint32_t* __restrict__ perm,
@@ -136,43 +189,25 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           const uint64_t* __restrict__ offsets,
                                                           uint8_t* __restrict__ dst)
 {
-  if (src.seg_sizes) {
-    __shared__ uint32_t part[256 / 64];
-    const uint64_t pairs = n_items * src.nseg;
-    for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x) {
-      const uint64_t item = pr / src.nseg;
-      const uint32_t seg = (uint32_t)(pr - item * src.nseg);
-      const uint32_t chunk = gather[item];
-      const uint32_t* sz = src.seg_sizes + (uint64_t)chunk * src.nseg;
-      const uint32_t nb = sz[seg];
-      if (nb == 0)
-        continue;  // uniform across the workgroup: no barrier is skipped by part of it
-      // a chunk's compressed bytes (below its slot stride) fit 32 bits
-      uint32_t before = 0;
-      for (uint32_t j = threadIdx.x; j < seg; j += 256)
-        before += sz[j];
-      before = wave_sum(before);
-      if ((threadIdx.x & 63) == 0)
-        part[threadIdx.x >> 6] = before;
-      __syncthreads();
-      before = part[0] + part[1] + part[2] + part[3];
-      __syncthreads();
-      block_copy(dst + offsets[perm[item]] + before,
-                 src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nb);
-    }
-    return;
-  }
+  __shared__ uint32_t part[256 / 64];
   const uint64_t pairs = n_items * src.nseg;
-  for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x) {
-    const uint64_t item = pr / src.nseg;
-    const uint32_t seg = (uint32_t)(pr - item * src.nseg);
-    const uint64_t seg_off = (uint64_t)seg * src.const_size;
-    const uint64_t nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
-    if (nbytes == 0 || seg_off >= src.const_total)
-      continue;
-    const uint32_t chunk = gather[item];
-    block_copy(dst + offsets[perm[item]] + seg_off,
-               src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nbytes);
+  for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x)
+    gather_pair(src, gather, perm, offsets, dst, pr, part);
+}
+
+// Every level's pairs, level after level, in one grid.
+__global__ __launch_bounds__(256) void aggregate_gather_levels_k(SegmentSource src, AggLevels a)
+{
+  __shared__ uint32_t part[256 / 64];
+  uint64_t first = 0;
+  for (int k = 0; k < a.nlev; ++k) {
+    const uint64_t pairs = a.n_items[k] * src.nseg;
+    // this workgroup's first pair of level k under the grid stride
+    uint64_t pr = first <= blockIdx.x ? blockIdx.x - first
+                                      : (gridDim.x - (first - blockIdx.x) % gridDim.x) % gridDim.x;
+    for (; pr < pairs; pr += gridDim.x)
+      gather_pair(src, a.gather[k], a.perm[k], a.offsets[k], a.dst[k], pr, part);
+    first += pairs;
   }
 }
 
@@ -186,6 +221,23 @@ hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gathe
 {
   hipLaunchKernelGGL(aggregate_plan_k, dim3(1), dim3(kPlanThreads), 0, stream, src, gather, perm,
                      n_items, n_slots, group, page_size, chunk_sizes, offsets, host_sizes, host_offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_aggregate_levels(const SegmentSource& src, const AggLevels& a, hipStream_t stream,
+                                   uint32_t max_workgroups)
+{
+  if (a.nlev <= 0 || a.nlev > kAggMaxLevels)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(aggregate_plan_levels_k, dim3((uint32_t)a.nlev), dim3(kPlanThreads), 0, stream, src, a);
+  uint64_t pairs = 0;
+  for (int k = 0; k < a.nlev; ++k)
+    pairs += a.n_items[k] * src.nseg;
+  if (pairs == 0)
+    return hipGetLastError();
+  const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
+  const uint64_t wg = pairs < cap ? pairs : cap;
+  hipLaunchKernelGGL(aggregate_gather_levels_k, dim3((uint32_t)wg), dim3(256), 0, stream, src, a);
   return hipGetLastError();
 }
 

@@ -50,6 +50,29 @@ hipError_t launch_aggregate_plan(const SegmentSource& src, const uint32_t* gathe
                                  uint64_t* offsets, hipStream_t stream, uint64_t* host_sizes = nullptr,
                                  uint64_t* host_offsets = nullptr);
 
+// Several levels' aggregates of one epoch: a plan workgroup per level, then
+// one gather grid over every level's (item, segment) pairs. Host mirrors of
+// the sizes and offsets are written by the plan kernel when given.
+constexpr int kAggMaxLevels = 16;
+
+struct AggLevels
+{
+  const uint32_t* gather[kAggMaxLevels];
+  const uint32_t* perm[kAggMaxLevels];
+  uint64_t n_items[kAggMaxLevels];
+  uint64_t n_slots[kAggMaxLevels];
+  uint64_t group[kAggMaxLevels];
+  uint64_t* sizes[kAggMaxLevels];
+  uint64_t* offsets[kAggMaxLevels];
+  uint64_t* host_sizes[kAggMaxLevels];
+  uint64_t* host_offsets[kAggMaxLevels];
+  uint8_t* dst[kAggMaxLevels];
+  int nlev;
+};
+
+hipError_t launch_aggregate_levels(const SegmentSource& src, const AggLevels& a, hipStream_t stream,
+                                   uint32_t max_workgroups);
+
 // Copy every item's segments to dst + offsets[perm[i]] (dst may be device or
 // pinned host memory).
 // max_workgroups caps the grid (0 = one workgroup per (item, segment));

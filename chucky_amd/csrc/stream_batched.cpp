@@ -53,6 +53,8 @@ struct BatchedState
   // each with its own work counter and share of the worker scratch.
   uint32_t NZ = 1;
   uint32_t zip_workers = 0;
+  // per-epoch compress and aggregate as captured graphs (else direct launches)
+  bool graphs = false;
   std::vector<hipStream_t> st_zip;
   uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
   bool dma = true; // gathered bytes reach host memory by DMA copies
@@ -187,6 +189,30 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
   const SegmentSource src = source_of(s);
   auto& P = b->pool[ps];
   auto& H = b->host[hs];
+  if (s->plan.nlod <= kAggMaxLevels) {
+    // every active level in two kernels (each graph node costs host time at
+    // every launch); the plan writes sizes and offsets to pinned memory too
+    AggLevels a = {};
+    for (int lv = 0; lv < s->plan.nlod; ++lv) {
+      if (!(mask & (1u << lv)))
+        continue;
+      const LevelInfo& L = s->plan.levels[lv];
+      const int k = a.nlev++;
+      a.gather[k] = P.d_gather[lv];
+      a.perm[k] = P.d_perm[lv];
+      a.n_items[k] = L.agg.chunks_per_epoch;
+      a.n_slots[k] = L.agg.covering_count;
+      a.group[k] = L.agg.cps_inner;
+      a.sizes[k] = P.d_sizes[lv];
+      a.offsets[k] = P.d_offsets[lv];
+      a.host_sizes[k] = H.h_sizes[lv];
+      a.host_offsets[k] = H.h_offsets[lv];
+      a.dst[k] = b->dma ? H.d_stage + H.level_base[lv] : H.h_agg + H.level_base[lv];
+    }
+    if (a.nlev)
+      BCHECK(launch_aggregate_levels(src, a, st, b->dma ? 0 : s->host_gather_wg));
+    return 0;
+  }
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     if (!(mask & (1u << lv)))
       continue;
@@ -350,6 +376,7 @@ int batched_init(tile_stream_gpu* s)
   BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
   BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
   b->dma = s->opts.batched_d2h != 1;
+  b->graphs = s->opts.batched_launch == 1;
   b->R = std::min<uint32_t>(K, 4);
   b->zip_workers = s->zworkers;
   if (p.config.codec.id == CODEC_ZSTD && !s->d_ztables) {
@@ -414,7 +441,7 @@ int batched_init(tile_stream_gpu* s)
     BCHECK(hipEventCreateWithFlags(&P.ev_compressed, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
     BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
-    if (p.config.codec.id != CODEC_NONE &&
+    if (b->graphs && p.config.codec.id != CODEC_NONE &&
         capture(s->st_comp, &P.compress_exec, [&](hipStream_t st) { return launch_compress(s, ps, st); }))
       return 1;
   }
@@ -496,7 +523,7 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
   }
   BCHECK(hipStreamWaitEvent(zs, P.ev_gathered, 0));
   hipEvent_t c0 = nullptr, c1 = nullptr;
-  if (P.compress_exec) {
+  if (s->plan.config.codec.id != CODEC_NONE) {
     c0 = take_timing_event(b);
     c1 = take_timing_event(b);
     if (!c0 || !c1) {
@@ -504,20 +531,27 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       return 1;
     }
     BCHECK(hipEventRecord(c0, zs));
-    BCHECK(hipGraphLaunch(P.compress_exec, zs));
+    if (b->graphs)
+      BCHECK(hipGraphLaunch(P.compress_exec, zs));
+    else if (launch_compress(s, pool_slot, zs))
+      return 1;
     BCHECK(hipEventRecord(c1, zs));
   }
   BCHECK(hipEventRecord(P.ev_compressed, zs));
   BCHECK(hipStreamWaitEvent(b->st_agg, P.ev_compressed, 0));
-  const auto key = std::make_tuple(pool_slot, hs, mask);
-  auto it = b->agg_exec.find(key);
-  if (it == b->agg_exec.end()) {
-    hipGraphExec_t ex = nullptr;
-    if (capture(b->st_agg, &ex, [&](hipStream_t st) { return launch_aggregate(s, b, pool_slot, hs, mask, st); }))
-      return 1;
-    it = b->agg_exec.emplace(key, ex).first;
+  if (b->graphs) {
+    const auto key = std::make_tuple(pool_slot, hs, mask);
+    auto it = b->agg_exec.find(key);
+    if (it == b->agg_exec.end()) {
+      hipGraphExec_t ex = nullptr;
+      if (capture(b->st_agg, &ex, [&](hipStream_t st) { return launch_aggregate(s, b, pool_slot, hs, mask, st); }))
+        return 1;
+      it = b->agg_exec.emplace(key, ex).first;
+    }
+    BCHECK(hipGraphLaunch(it->second, b->st_agg));
+  } else if (launch_aggregate(s, b, pool_slot, hs, mask, b->st_agg)) {
+    return 1;
   }
-  BCHECK(hipGraphLaunch(it->second, b->st_agg));
   BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
   hipEvent_t done = take_event(b);
   if (!done) {

@@ -37,6 +37,8 @@ struct tile_stream_gpu_options
   uint32_t zstd_chunk_tables;      /* one Huffman table per chunk: 0 = autotuned, 1 = on, 2 = off */
   uint32_t batched_d2h;            /* batched pipeline: 0/2 = gather on the device + DMA copy,
                                       1 = gather kernel writing pinned host memory */
+  uint32_t batched_launch;         /* batched pipeline per-epoch work: 0 = autotuned,
+                                      1 = captured graphs, 2 = direct kernel launches */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);

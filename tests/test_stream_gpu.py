@@ -187,10 +187,13 @@ BATCHED_CASES = {
 
 @pytest.mark.parametrize("name", sorted(BATCHED_CASES))
 @pytest.mark.parametrize("device_input", [False, True])
-def test_batched_pipeline_matches_reference(name, device_input):
+@pytest.mark.parametrize("launch", [1, 2])
+def test_batched_pipeline_matches_reference(name, device_input, launch):
+    # launch 1: per-epoch compress and aggregate as captured graphs; 2: direct launches
     make, dims, dtype, codec, extra = BATCHED_CASES[name]
     cfg = ca.Config(dims, dtype=dtype, codec=codec, **extra)
-    _compare(cfg, make(), pieces=3, device_input=device_input, pipeline=ca.PIPELINE_BATCHED)
+    _compare(cfg, make(), pieces=3, device_input=device_input, pipeline=ca.PIPELINE_BATCHED,
+             batched_launch=launch)
 
 
 @pytest.mark.parametrize("rows,K,codec", [(40, 4, "none"), (40, 8, "lz4"), (12, 4, "none"),
