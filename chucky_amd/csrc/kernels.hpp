@@ -87,7 +87,9 @@ size_t lz4_max_output_size(size_t chunk_bytes);
 // zstd block sizes: powers of two in [kZstdBlockMin, kZstdBlockMax]; each
 // block is compressed by one wavefront, so the block size sets the number of
 // wavefronts per chunk.
-constexpr uint32_t kZstdBlockMin = 8 * 1024;
+constexpr uint32_t kZstdBlockMin = 4 * 1024;
+// blocks up to this size are staged whole in LDS by their worker
+constexpr uint32_t kZstdStageBytes = 8 * 1024;
 constexpr uint32_t kZstdBlockMax = 128 * 1024;
 bool zstd_block_ok(uint32_t block);
 size_t zstd_max_output_size(size_t chunk_bytes, uint32_t block);

@@ -1335,7 +1335,7 @@ __global__ __launch_bounds__(256) void zstd_table_k(const uint8_t* __restrict__ 
     t.tree_size = tree_size;
 }
 
-// LDS of one worker. Staged workers (blocks of at most kZstdBlockMin bytes)
+// LDS of one worker. Staged workers (blocks of at most kZstdStageBytes)
 // also hold their block's input, so the parse and the literal passes read LDS
 // instead of global memory: 18.3 KiB a wavefront, 8 per CU, for the short
 // launches the smallest block size is chosen for.
@@ -1346,7 +1346,7 @@ template<bool Staged> struct WorkerLds
 template<> struct WorkerLds<true>
 {
   Lds L;
-  alignas(16) uint8_t in[kZstdBlockMin + 64];
+  alignas(16) uint8_t in[kZstdStageBytes + 64];
 };
 
 template<bool Staged>
@@ -1508,7 +1508,7 @@ hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_b
   }
   const dim3 grid(queue_counter ? workers : (uint32_t)total);
   // staged workers need 16-byte aligned block starts
-  const bool staged = block <= kZstdBlockMin && (src_stride % 16) == 0 && ((uintptr_t)src % 16) == 0;
+  const bool staged = block <= kZstdStageBytes && (src_stride % 16) == 0 && ((uintptr_t)src % 16) == 0;
   if (staged)
     hipLaunchKernelGGL(zstd_k<true>, grid, dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
                        block, nseg, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,

@@ -234,7 +234,8 @@ ZSTD_MODE_DATA = {
 
 
 @pytest.mark.parametrize("name", sorted(ZSTD_MODE_DATA))
-@pytest.mark.parametrize("block,tables", [(8192, 1), (8192, 2), (32768, 1), (131072, 1), (131072, 2)])
+@pytest.mark.parametrize("block,tables", [(4096, 2), (8192, 1), (8192, 2), (32768, 1), (131072, 1),
+                                          (131072, 2)])
 def test_zstd_block_sizes_and_table_modes(name, block, tables):
     # per-block Huffman tables (2) or one table per chunk carried by the first
     # block and reused as treeless literals (1), at several block sizes
