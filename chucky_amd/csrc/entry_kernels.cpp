@@ -54,7 +54,7 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
     if (zstd_prepare() != hipSuccess)
       goto done;
     if (hipMalloc(&scratch, zstd_scratch_bytes(workers)) != hipSuccess ||
-        hipMalloc((void**)&queue, 256) != hipSuccess)
+        hipMalloc((void**)&queue, 256) != hipSuccess || hipMemsetAsync(queue, 0, 256, st) != hipSuccess)
       goto done;
     if (launch_zstd((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
                     seg, level, kZstdBlockMax, scratch, workers, queue, nullptr, st) != hipSuccess)

@@ -108,7 +108,11 @@ hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_by
 // Uploads the predefined FSE tables to the current device (once per device;
 // call before the first launch_zstd on a device, outside graph capture).
 hipError_t zstd_prepare();
-// scratch: see zstd_scratch_bytes(workers).
+// scratch: see zstd_scratch_bytes(workers). queue_counter (two u32, zero
+// before the first launch on them, e.g. zeroed with their allocation): the
+// persistent workers take blocks from it and the last worker out sets both
+// words back to zero, so launches on one counter need no clearing between
+// them. Without it, one workgroup per block (blocks <= workers).
 size_t zstd_scratch_bytes(uint32_t workers);
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,

@@ -29,7 +29,6 @@
 #include "kernels.hpp"
 #include "stream_gpu.hpp"
 
-#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -660,15 +659,20 @@ inline dim3 grid_of(uint64_t n) { return dim3((uint32_t)((n + 255) / 256)); }
 
 inline uint32_t batch_grid(uint64_t nb) { return (uint32_t)std::min<uint64_t>(nb, 65535); }
 
-// A launch that records the timing events it is given: `start` when the
-// kernel begins, `stop` when it ends (either may be null).
+// A launch that records the timing events it is given (either may be null):
+// `start` on the stream right before the kernel, `stop` right after it, so
+// the pair brackets the kernel's execution. (hipExtLaunchKernelGGL's
+// in-kernel events measure the same span for about twice the host time per
+// launch; a host-paced pipeline of small epochs feels that.) A failed record
+// leaves its error for the caller's hipGetLastError.
 template<typename K, typename... A>
 void launch(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t start, hipEvent_t stop, A... args)
 {
-  if (start || stop)
-    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, start, stop, 0, args...);
-  else
-    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+  if (start)
+    (void)hipEventRecord(start, st);
+  hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+  if (stop)
+    (void)hipEventRecord(stop, st);
 }
 
 TileGeom geom_of(const LodTiles& t)

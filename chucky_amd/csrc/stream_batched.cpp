@@ -386,6 +386,9 @@ int batched_init(tile_stream_gpu* s)
   }
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
     return 1;
+  // the compressors' work counters start at zero and each launch leaves them so
+  BCHECK(hipMemsetAsync(b->d_queues, 0, 16 * b->NZ * sizeof(uint32_t), b->st_agg));
+  BCHECK(hipStreamSynchronize(b->st_agg));
   b->st_zip.assign(b->NZ, nullptr);
   for (auto& z : b->st_zip)
     BCHECK(hipStreamCreateWithFlags(&z, hipStreamNonBlocking));

@@ -758,6 +758,8 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
       return 1;
     if (dalloc(s, (void**)&s->d_queue, 64))
       return 1;
+    // the compressor's work counter starts at zero and each launch leaves it so
+    HIPCHECK(hipMemsetAsync(s->d_queue, 0, 64, s->st_comp));
   }
   if (config->codec.id != CODEC_NONE) {
     if (dalloc(s, (void**)&s->d_comp, nchunks * s->chunk_slot))

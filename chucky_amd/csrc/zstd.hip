@@ -1411,9 +1411,14 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
     if (!queue)
       break;
   }
+  // queue[0] hands out blocks, queue[1] counts the workers that found none
+  // left; the last of them clears both for the next launch on this queue
+  // (every worker has taken its last item by then)
+  if (queue && lane == 0 && atomicAdd(queue + 1, 1u) == gridDim.x - 1) {
+    atomicExch(queue + 1, 0u);
+    atomicExch(queue, 0u);
+  }
 }
-
-__global__ void zero_u32_k(uint32_t* p) { *p = 0; }
 
 std::mutex g_pre_mu;
 std::map<int, zcore::Predefined*> g_pre; // per device, kept until the process ends
@@ -1501,11 +1506,8 @@ hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_b
     if (e != hipSuccess)
       return e;
   }
-  if (queue_counter) {
-    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(1), 0, stream, queue_counter);
-  } else if (total > workers) {
+  if (!queue_counter && total > workers)
     return hipErrorInvalidValue; // one scratch slice per block in this mode
-  }
   const dim3 grid(queue_counter ? workers : (uint32_t)total);
   // staged workers need 16-byte aligned block starts
   const bool staged = block <= kZstdStageBytes && (src_stride % 16) == 0 && ((uintptr_t)src % 16) == 0;
