@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace chucky {
 
@@ -43,6 +44,10 @@ namespace {
   } while (0)
 
 } // namespace
+
+// compression queues at most: with the compute and copy streams they still
+// fit HIP's 4 hardware queues
+constexpr uint32_t kMaxZip = 2;
 
 struct BatchedState
 {
@@ -304,22 +309,22 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     return 0;
   auto& t = b->inflight.front();
   if (b->dma) {
-    if (issue_ready_copies(s, b))
-      return 1;
-    if (!t.copied) {
-      if (!block)
-        return 2;
-      BCHECK(hipEventSynchronize(t.done));
+    // poll rather than block: while the oldest epoch's copy runs, the copies
+    // of later epochs are queued as soon as their gathers finish, so the DMA
+    // engine never waits for this thread to wake up
+    for (;;) {
       if (issue_ready_copies(s, b))
         return 1;
-    }
-    if (block) {
-      BCHECK(hipEventSynchronize(t.copied));
-    } else {
-      const hipError_t q = hipEventQuery(t.copied);
-      if (q == hipErrorNotReady)
+      if (t.copied) {
+        const hipError_t q = hipEventQuery(t.copied);
+        if (q == hipSuccess)
+          break;
+        if (q != hipErrorNotReady)
+          BCHECK(q);
+      }
+      if (!block)
         return 2;
-      BCHECK(q);
+      std::this_thread::yield();
     }
     b->event_pool.push_back(t.copied);
     t.copied = nullptr;
@@ -373,25 +378,35 @@ int batched_init(tile_stream_gpu* s)
   s->batched = b;
   const Plan& p = s->plan;
   const uint32_t K = p.epochs_per_batch;
-  BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
-  BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
   b->dma = s->opts.batched_d2h != 1;
   b->graphs = s->opts.batched_launch == 1;
   b->R = std::min<uint32_t>(K, 4);
   b->zip_workers = s->zworkers;
   if (p.config.codec.id == CODEC_ZSTD && !s->d_ztables) {
     const uint64_t blocks = std::max<uint64_t>((uint64_t)p.total_chunks * s->nseg, 1);
-    b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, 4), std::max<uint64_t>(1, s->zworkers / blocks));
+    b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, kMaxZip), std::max<uint64_t>(1, s->zworkers / blocks));
     b->zip_workers = (uint32_t)std::min<uint64_t>(blocks, s->zworkers / b->NZ);
   }
+  // HIP maps streams onto its hardware queues (GPU_MAX_HW_QUEUES, 4 by
+  // default) in creation order, and kernels of streams sharing a queue run
+  // in submission order. The streams that launch kernels are created right
+  // after the compute stream, so that they land on distinct queues: the
+  // compression queues (each also runs its epochs' aggregates) or, with one
+  // compression queue (the compute stream itself), the aggregate stream; the
+  // copy-only stream follows.
+  if (b->NZ > 1) {
+    b->st_zip.assign(b->NZ, nullptr);
+    for (auto& z : b->st_zip)
+      BCHECK(hipStreamCreateWithFlags(&z, hipStreamNonBlocking));
+  } else {
+    BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
+  }
+  BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
     return 1;
   // the compressors' work counters start at zero and each launch leaves them so
-  BCHECK(hipMemsetAsync(b->d_queues, 0, 16 * b->NZ * sizeof(uint32_t), b->st_agg));
-  BCHECK(hipStreamSynchronize(b->st_agg));
-  b->st_zip.assign(b->NZ, nullptr);
-  for (auto& z : b->st_zip)
-    BCHECK(hipStreamCreateWithFlags(&z, hipStreamNonBlocking));
+  BCHECK(hipMemsetAsync(b->d_queues, 0, 16 * b->NZ * sizeof(uint32_t), s->st_comp));
+  BCHECK(hipStreamSynchronize(s->st_comp));
   // worst-case bytes of one epoch per level
   std::vector<uint64_t> level_bytes(p.nlod);
   uint64_t total = 0;
@@ -443,7 +458,7 @@ int batched_init(tile_stream_gpu* s)
     BCHECK(hipEventCreateWithFlags(&P.ev_filled, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_compressed, hipEventDisableTiming));
     BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
-    BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
+    BCHECK(hipEventRecord(P.ev_gathered, s->st_comp));
     if (b->graphs && p.config.codec.id != CODEC_NONE &&
         capture(s->st_comp, &P.compress_exec, [&](hipStream_t st) { return launch_compress(s, ps, st); }))
       return 1;
@@ -541,27 +556,31 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     BCHECK(hipEventRecord(c1, zs));
   }
   BCHECK(hipEventRecord(P.ev_compressed, zs));
-  BCHECK(hipStreamWaitEvent(b->st_agg, P.ev_compressed, 0));
+  // the aggregate runs on the compression queue after the compress, or on its
+  // own stream when the compute stream compresses
+  hipStream_t as = b->NZ > 1 ? zs : b->st_agg;
+  if (as != zs)
+    BCHECK(hipStreamWaitEvent(as, P.ev_compressed, 0));
   if (b->graphs) {
     const auto key = std::make_tuple(pool_slot, hs, mask);
     auto it = b->agg_exec.find(key);
     if (it == b->agg_exec.end()) {
       hipGraphExec_t ex = nullptr;
-      if (capture(b->st_agg, &ex, [&](hipStream_t st) { return launch_aggregate(s, b, pool_slot, hs, mask, st); }))
+      if (capture(as, &ex, [&](hipStream_t st) { return launch_aggregate(s, b, pool_slot, hs, mask, st); }))
         return 1;
       it = b->agg_exec.emplace(key, ex).first;
     }
-    BCHECK(hipGraphLaunch(it->second, b->st_agg));
-  } else if (launch_aggregate(s, b, pool_slot, hs, mask, b->st_agg)) {
+    BCHECK(hipGraphLaunch(it->second, as));
+  } else if (launch_aggregate(s, b, pool_slot, hs, mask, as)) {
     return 1;
   }
-  BCHECK(hipEventRecord(P.ev_gathered, b->st_agg));
+  BCHECK(hipEventRecord(P.ev_gathered, as));
   hipEvent_t done = take_event(b);
   if (!done) {
     set_error("batched: event creation failed");
     return 1;
   }
-  BCHECK(hipEventRecord(done, b->st_agg));
+  BCHECK(hipEventRecord(done, as));
   b->inflight.push_back({ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
                           std::chrono::steady_clock::now() });
   // opportunistic in-order delivery of whatever has finished
@@ -598,7 +617,8 @@ int batched_drain(tile_stream_gpu* s)
   BCHECK(hipStreamSynchronize(s->st_comp));
   for (auto z : b->st_zip)
     BCHECK(hipStreamSynchronize(z));
-  BCHECK(hipStreamSynchronize(b->st_agg));
+  if (b->st_agg)
+    BCHECK(hipStreamSynchronize(b->st_agg));
   BCHECK(hipStreamSynchronize(b->st_copy));
   scatter_collect_ready(s);
   return 0;

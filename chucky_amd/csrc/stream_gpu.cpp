@@ -691,7 +691,6 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   HIPCHECK(hipGetDevice(&s->device));
   HIPCHECK(hipStreamCreateWithFlags(&s->st_h2d, hipStreamNonBlocking));
   HIPCHECK(hipStreamCreateWithFlags(&s->st_comp, hipStreamNonBlocking));
-  HIPCHECK(hipStreamCreateWithFlags(&s->st_d2h, hipStreamNonBlocking));
 
   const uint32_t K = p.epochs_per_batch;
   s->host_gather_wg = s->opts.host_gather_workgroups ? s->opts.host_gather_workgroups
@@ -826,6 +825,10 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   // only without shard page alignment; aligned streams stay on the plain path.
   if (s->pipeline == CHUCKY_PIPELINE_BATCHED && config->shard_alignment == 0 && batched_init(s))
     return 1;
+  // created last: a copy-only stream, while the streams that launch kernels
+  // (compute, then the batched pipeline's) take HIP's hardware queues in
+  // creation order
+  HIPCHECK(hipStreamCreateWithFlags(&s->st_d2h, hipStreamNonBlocking));
   HIPCHECK(hipStreamSynchronize(s->st_comp));
   return 0;
 }
