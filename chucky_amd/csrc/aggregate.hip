@@ -199,15 +199,18 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
 __global__ __launch_bounds__(256) void aggregate_gather_levels_k(SegmentSource src, AggLevels a)
 {
   __shared__ uint32_t part[256 / 64];
-  uint64_t first = 0;
+  uint64_t first = 0, base = 0;
   for (int k = 0; k < a.nlev; ++k) {
     const uint64_t pairs = a.n_items[k] * src.nseg;
     // this workgroup's first pair of level k under the grid stride
     uint64_t pr = first <= blockIdx.x ? blockIdx.x - first
                                       : (gridDim.x - (first - blockIdx.x) % gridDim.x) % gridDim.x;
+    uint8_t* dst = a.packed ? a.dst[0] + base : a.dst[k];
     for (; pr < pairs; pr += gridDim.x)
-      gather_pair(src, a.gather[k], a.perm[k], a.offsets[k], a.dst[k], pr, part);
+      gather_pair(src, a.gather[k], a.perm[k], a.offsets[k], dst, pr, part);
     first += pairs;
+    if (a.packed)
+      base += a.offsets[k][a.n_slots[k]]; // level k's total, from the plan launch
   }
 }
 

@@ -68,6 +68,9 @@ struct AggLevels
   uint64_t* host_offsets[kAggMaxLevels];
   uint8_t* dst[kAggMaxLevels];
   int nlev;
+  // packed: every level's bytes follow the previous level's in dst[0] (one
+  // copy moves the whole epoch); dst[k] is then ignored for k > 0
+  int packed;
 };
 
 hipError_t launch_aggregate_levels(const SegmentSource& src, const AggLevels& a, hipStream_t stream,

@@ -214,6 +214,8 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
       a.host_offsets[k] = H.h_offsets[lv];
       a.dst[k] = b->dma ? H.d_stage + H.level_base[lv] : H.h_agg + H.level_base[lv];
     }
+    // DMA mode packs the levels back to back, so one copy moves the epoch
+    a.packed = b->dma ? 1 : 0;
     if (a.nlev)
       BCHECK(launch_aggregate_levels(src, a, st, b->dma ? 0 : s->host_gather_wg));
     return 0;
@@ -263,17 +265,42 @@ int capture(hipStream_t st, hipGraphExec_t* exec, F fn)
 
 // DMA mode: queue the copies of a gathered epoch (its offsets are in pinned
 // memory once the aggregate graph has finished).
+// Byte offset of level lv's bytes in a host slot for an epoch with `mask`:
+// packed (DMA mode, at most kAggMaxLevels levels) right after the previous
+// active levels' bytes, else at the level's worst-case base.
+uint64_t level_offset(const tile_stream_gpu* s, const BatchedState* b, const BatchedState::HostSlot& H,
+                      uint32_t mask, int lv)
+{
+  if (!b->dma || s->plan.nlod > kAggMaxLevels)
+    return H.level_base[lv];
+  uint64_t off = 0;
+  for (int j = 0; j < lv; ++j)
+    if (mask & (1u << j))
+      off += H.h_offsets[j][s->plan.levels[j].agg.covering_count];
+  return off;
+}
+
 int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t)
 {
   auto& H = b->host[t.host_slot];
-  for (int lv = 0; lv < s->plan.nlod; ++lv) {
-    if (!(t.mask & (1u << lv)))
-      continue;
-    const uint64_t C = s->plan.levels[lv].agg.covering_count;
-    const uint64_t bytes = H.h_offsets[lv][C];
+  if (s->plan.nlod <= kAggMaxLevels) {
+    // the levels sit back to back: one copy
+    uint64_t bytes = 0;
+    for (int lv = 0; lv < s->plan.nlod; ++lv)
+      if (t.mask & (1u << lv))
+        bytes += H.h_offsets[lv][s->plan.levels[lv].agg.covering_count];
     if (bytes)
-      BCHECK(hipMemcpyAsync(H.h_agg + H.level_base[lv], H.d_stage + H.level_base[lv], bytes,
-                            hipMemcpyDeviceToHost, b->st_copy));
+      BCHECK(hipMemcpyAsync(H.h_agg, H.d_stage, bytes, hipMemcpyDeviceToHost, b->st_copy));
+  } else {
+    for (int lv = 0; lv < s->plan.nlod; ++lv) {
+      if (!(t.mask & (1u << lv)))
+        continue;
+      const uint64_t C = s->plan.levels[lv].agg.covering_count;
+      const uint64_t bytes = H.h_offsets[lv][C];
+      if (bytes)
+        BCHECK(hipMemcpyAsync(H.h_agg + H.level_base[lv], H.d_stage + H.level_base[lv], bytes,
+                              hipMemcpyDeviceToHost, b->st_copy));
+    }
   }
   t.copied = take_event(b);
   if (!t.copied) {
@@ -351,7 +378,7 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     if (!(t.mask & (1u << lv)))
       continue;
-    AggView v{ H.h_agg + H.level_base[lv], H.h_offsets[lv], H.h_sizes[lv] };
+    AggView v{ H.h_agg + level_offset(s, b, H, t.mask, lv), H.h_offsets[lv], H.h_sizes[lv] };
     size_t out = 0;
     if (deliver_batch((uint8_t)lv, s->shards[lv], v, 1, s->sink, 0, &out))
       return 1;
