@@ -93,6 +93,10 @@ struct BatchedState
     hipEvent_t copied; // DMA copies finished (DMA mode), else null
     hipEvent_t c0, c1; // compress graph start / end (timing events)
     std::chrono::steady_clock::time_point t0;
+    // split epochs: level 0 (mask_a) is compressed, gathered and copied on
+    // its own, beside the coarse levels (done / copied)
+    uint32_t mask_a = 0;
+    hipEvent_t done_a = nullptr, copied_a = nullptr;
   };
   std::deque<Ticket> inflight;
   uint64_t next_epoch = 0;
@@ -188,8 +192,27 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
   return 0;
 }
 
+// zstd on chunks [c0, c1) of pool slot `slot`, with compression queue z's
+// work counter and scratch share, on stream st.
+int launch_zstd_range(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0, uint64_t c1, hipStream_t st)
+{
+  const BatchedState* b = s->batched;
+  const uint64_t T = s->plan.total_chunks;
+  if (c1 <= c0)
+    return 0;
+  BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes + c0 * s->chunk_stride_bytes,
+                     s->chunk_stride_bytes, s->chunk_bytes, c1 - c0,
+                     s->d_comp + ((uint64_t)slot * T + c0) * s->chunk_slot, s->chunk_slot,
+                     s->d_seg_sizes + ((uint64_t)slot * T + c0) * s->nseg, s->plan.config.codec.level, s->zblock,
+                     (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers), b->zip_workers,
+                     b->d_queues + 16 * z, nullptr, st));
+  return 0;
+}
+
+// Aggregate of the levels in `mask`; in DMA mode their bytes are packed from
+// d_stage + base on.
 int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t hs, uint32_t mask,
-                     hipStream_t st)
+                     hipStream_t st, uint64_t base = 0)
 {
   const SegmentSource src = source_of(s);
   auto& P = b->pool[ps];
@@ -212,7 +235,7 @@ int launch_aggregate(tile_stream_gpu* s, BatchedState* b, uint32_t ps, uint32_t 
       a.offsets[k] = P.d_offsets[lv];
       a.host_sizes[k] = H.h_sizes[lv];
       a.host_offsets[k] = H.h_offsets[lv];
-      a.dst[k] = b->dma ? H.d_stage + H.level_base[lv] : H.h_agg + H.level_base[lv];
+      a.dst[k] = b->dma ? H.d_stage + base : H.h_agg + H.level_base[lv];
     }
     // DMA mode packs the levels back to back, so one copy moves the epoch
     a.packed = b->dma ? 1 : 0;
@@ -269,31 +292,38 @@ int capture(hipStream_t st, hipGraphExec_t* exec, F fn)
 // packed (DMA mode, at most kAggMaxLevels levels) right after the previous
 // active levels' bytes, else at the level's worst-case base.
 uint64_t level_offset(const tile_stream_gpu* s, const BatchedState* b, const BatchedState::HostSlot& H,
-                      uint32_t mask, int lv)
+                      const BatchedState::Ticket& t, int lv)
 {
   if (!b->dma || s->plan.nlod > kAggMaxLevels)
     return H.level_base[lv];
-  uint64_t off = 0;
-  for (int j = 0; j < lv; ++j)
-    if (mask & (1u << j))
+  // a split epoch keeps level 0 at 0 and packs the coarse levels from
+  // level 1's base on
+  const int from = t.mask_a && lv > 0 ? 1 : 0;
+  uint64_t off = from ? H.level_base[1] : 0;
+  for (int j = from; j < lv; ++j)
+    if (t.mask & (1u << j))
       off += H.h_offsets[j][s->plan.levels[j].agg.covering_count];
   return off;
 }
 
-int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t)
+// Queues the DMA copy of one part of a ticket: level 0 of a split epoch
+// (part_a), else the rest (every level of an unsplit epoch).
+int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t, bool part_a)
 {
   auto& H = b->host[t.host_slot];
+  const uint32_t mask = part_a ? t.mask_a : (t.mask & ~t.mask_a);
   if (s->plan.nlod <= kAggMaxLevels) {
-    // the levels sit back to back: one copy
+    // the part's levels sit back to back: one copy
+    const uint64_t base = !part_a && t.mask_a ? H.level_base[1] : 0;
     uint64_t bytes = 0;
     for (int lv = 0; lv < s->plan.nlod; ++lv)
-      if (t.mask & (1u << lv))
+      if (mask & (1u << lv))
         bytes += H.h_offsets[lv][s->plan.levels[lv].agg.covering_count];
     if (bytes)
-      BCHECK(hipMemcpyAsync(H.h_agg, H.d_stage, bytes, hipMemcpyDeviceToHost, b->st_copy));
+      BCHECK(hipMemcpyAsync(H.h_agg + base, H.d_stage + base, bytes, hipMemcpyDeviceToHost, b->st_copy));
   } else {
     for (int lv = 0; lv < s->plan.nlod; ++lv) {
-      if (!(t.mask & (1u << lv)))
+      if (!(mask & (1u << lv)))
         continue;
       const uint64_t C = s->plan.levels[lv].agg.covering_count;
       const uint64_t bytes = H.h_offsets[lv][C];
@@ -302,12 +332,13 @@ int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t)
                               hipMemcpyDeviceToHost, b->st_copy));
     }
   }
-  t.copied = take_event(b);
-  if (!t.copied) {
+  hipEvent_t& ev = part_a ? t.copied_a : t.copied;
+  ev = take_event(b);
+  if (!ev) {
     set_error("batched: event creation failed");
     return 1;
   }
-  BCHECK(hipEventRecord(t.copied, b->st_copy));
+  BCHECK(hipEventRecord(ev, b->st_copy));
   return 0;
 }
 
@@ -316,13 +347,21 @@ int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t)
 int issue_ready_copies(tile_stream_gpu* s, BatchedState* b)
 {
   for (auto& t : b->inflight) {
+    if (t.mask_a && !t.copied_a) {
+      const hipError_t q = hipEventQuery(t.done_a);
+      if (q == hipErrorNotReady)
+        return 0;
+      BCHECK(q);
+      if (issue_copies(s, b, t, true))
+        return 1;
+    }
     if (t.copied)
       continue;
     const hipError_t q = hipEventQuery(t.done);
     if (q == hipErrorNotReady)
       return 0;
     BCHECK(q);
-    if (issue_copies(s, b, t))
+    if (issue_copies(s, b, t, false))
       return 1;
   }
   return 0;
@@ -342,8 +381,10 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     for (;;) {
       if (issue_ready_copies(s, b))
         return 1;
-      if (t.copied) {
-        const hipError_t q = hipEventQuery(t.copied);
+      if (t.copied && (!t.mask_a || t.copied_a)) {
+        hipError_t q = hipEventQuery(t.copied);
+        if (q == hipSuccess && t.mask_a)
+          q = hipEventQuery(t.copied_a);
         if (q == hipSuccess)
           break;
         if (q != hipErrorNotReady)
@@ -355,6 +396,11 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     }
     b->event_pool.push_back(t.copied);
     t.copied = nullptr;
+    if (t.mask_a) {
+      b->event_pool.push_back(t.copied_a);
+      b->event_pool.push_back(t.done_a);
+      t.copied_a = t.done_a = nullptr;
+    }
   } else if (block) {
     BCHECK(hipEventSynchronize(t.done));
   } else {
@@ -378,7 +424,7 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
     if (!(t.mask & (1u << lv)))
       continue;
-    AggView v{ H.h_agg + level_offset(s, b, H, t.mask, lv), H.h_offsets[lv], H.h_sizes[lv] };
+    AggView v{ H.h_agg + level_offset(s, b, H, t, lv), H.h_offsets[lv], H.h_sizes[lv] };
     size_t out = 0;
     if (deliver_batch((uint8_t)lv, s->shards[lv], v, 1, s->sink, 0, &out))
       return 1;
@@ -558,6 +604,56 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
   auto& P = b->pool[pool_slot];
   s->slot_compressed[pool_slot] = 1;
+  const Plan& p = s->plan;
+  // An epoch that also emits coarse levels is split over both compression
+  // queues: the coarse levels' chunks (their blocks are the slow ones) start
+  // first on the slot's queue, level 0 beside them on the other queue, and
+  // level 0's gather and copy leave while the coarse levels still compress.
+  if (b->NZ > 1 && b->dma && !b->graphs && p.nlod > 1 && p.nlod <= kAggMaxLevels &&
+      p.config.codec.id == CODEC_ZSTD && (mask & 1u) && (mask & ~1u)) {
+    hipStream_t zb = b->st_zip[pool_slot % b->NZ], za = b->st_zip[(pool_slot + 1) % b->NZ];
+    const uint64_t n0 = p.levels[1].chunk_offset; // level 0 chunks come first
+    BCHECK(hipEventRecord(P.ev_filled, s->st_comp));
+    for (hipStream_t z : { zb, za }) {
+      BCHECK(hipStreamWaitEvent(z, P.ev_filled, 0));
+      BCHECK(hipStreamWaitEvent(z, P.ev_gathered, 0));
+    }
+    hipEvent_t c0 = take_timing_event(b), c1 = take_timing_event(b);
+    hipEvent_t done_a = take_event(b), done = take_event(b);
+    if (!c0 || !c1 || !done_a || !done) {
+      set_error("batched: event creation failed");
+      return 1;
+    }
+    BCHECK(hipEventRecord(c0, zb));
+    if (launch_zstd_range(s, pool_slot, pool_slot % b->NZ, n0, p.total_chunks, zb) ||
+        launch_zstd_range(s, pool_slot, (pool_slot + 1) % b->NZ, 0, n0, za))
+      return 1;
+    BCHECK(hipEventRecord(P.ev_compressed, za));
+    if (launch_aggregate(s, b, pool_slot, hs, 1u, za, 0))
+      return 1;
+    BCHECK(hipEventRecord(done_a, za));
+    // the compress time spans both launches
+    BCHECK(hipStreamWaitEvent(zb, P.ev_compressed, 0));
+    BCHECK(hipEventRecord(c1, zb));
+    if (launch_aggregate(s, b, pool_slot, hs, mask & ~1u, zb, H.level_base[1]))
+      return 1;
+    BCHECK(hipStreamWaitEvent(zb, done_a, 0));
+    BCHECK(hipEventRecord(P.ev_gathered, zb));
+    BCHECK(hipEventRecord(done, zb));
+    BatchedState::Ticket t{ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
+                            std::chrono::steady_clock::now() };
+    t.mask_a = 1u;
+    t.done_a = done_a;
+    b->inflight.push_back(t);
+    for (;;) {
+      const int r = deliver_one(s, b, false);
+      if (r == 1)
+        return 1;
+      if (r == 2 || b->inflight.empty())
+        break;
+    }
+    return 0;
+  }
   // the slot's compression queue takes over once the epoch is in the pool;
   // compressed slots of this pool slot are free once its last gather is done
   hipStream_t zs = s->st_comp;
