@@ -28,6 +28,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 
 namespace chucky {
@@ -101,6 +103,14 @@ struct BatchedState
   std::deque<Ticket> inflight;
   uint64_t next_epoch = 0;
   std::vector<hipEvent_t> event_pool, timing_pool;
+  // DMA mode: a copier thread waits for each gather in epoch order and
+  // queues its copy at once, whatever the submitting thread is doing. The
+  // mutex guards inflight, the event pools and the tickets' copy events.
+  std::recursive_mutex mu;
+  std::condition_variable_any cv; // a ticket was added, or stop
+  std::thread copier;
+  bool stop = false;
+  std::string copy_error; // first failure of the copier thread
 };
 
 namespace {
@@ -148,6 +158,7 @@ int host_alloc(tile_stream_gpu* s, T** p, size_t count)
 
 hipEvent_t take_event(BatchedState* b)
 {
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
   if (!b->event_pool.empty()) {
     hipEvent_t e = b->event_pool.back();
     b->event_pool.pop_back();
@@ -161,6 +172,7 @@ hipEvent_t take_event(BatchedState* b)
 
 hipEvent_t take_timing_event(BatchedState* b)
 {
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
   if (!b->timing_pool.empty()) {
     hipEvent_t e = b->timing_pool.back();
     b->timing_pool.pop_back();
@@ -342,29 +354,46 @@ int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t, b
   return 0;
 }
 
-// DMA mode: queue copies for every in-flight epoch whose gather has finished
-// (in order, without blocking), so the copy engine always has work.
-int issue_ready_copies(tile_stream_gpu* s, BatchedState* b)
+// The copier thread: the next part without a copy (in epoch order) is
+// waited for and its copy queued; sleeps while there is none.
+void copier_loop(tile_stream_gpu* s, BatchedState* b)
 {
-  for (auto& t : b->inflight) {
-    if (t.mask_a && !t.copied_a) {
-      const hipError_t q = hipEventQuery(t.done_a);
-      if (q == hipErrorNotReady)
-        return 0;
-      BCHECK(q);
-      if (issue_copies(s, b, t, true))
-        return 1;
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::recursive_mutex> lk(b->mu);
+  for (;;) {
+    BatchedState::Ticket* t = nullptr;
+    bool part_a = false;
+    for (auto& x : b->inflight) {
+      if (x.mask_a && !x.copied_a) {
+        t = &x;
+        part_a = true;
+        break;
+      }
+      if (!x.copied) {
+        t = &x;
+        break;
+      }
     }
-    if (t.copied)
+    if (!t) {
+      if (b->stop)
+        return;
+      b->cv.wait(lk);
       continue;
-    const hipError_t q = hipEventQuery(t.done);
-    if (q == hipErrorNotReady)
-      return 0;
-    BCHECK(q);
-    if (issue_copies(s, b, t, false))
-      return 1;
+    }
+    const hipEvent_t ev = part_a ? t->done_a : t->done;
+    lk.unlock();
+    const hipError_t e = hipEventSynchronize(ev);
+    lk.lock();
+    // the ticket stays in flight until its copies are done, so t is valid
+    if (e != hipSuccess || issue_copies(s, b, *t, part_a)) {
+      // the submitting thread reports it at its next delivery; no further copies
+      b->copy_error = std::string("batched: copy thread: ") +
+                      (e != hipSuccess ? hipGetErrorString(e) : chucky_last_error());
+      while (!b->stop)
+        b->cv.wait(lk);
+      return;
+    }
   }
-  return 0;
 }
 
 // Hands the oldest in-flight epoch to the sink once its bytes are in host
@@ -375,12 +404,13 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     return 0;
   auto& t = b->inflight.front();
   if (b->dma) {
-    // poll rather than block: while the oldest epoch's copy runs, the copies
-    // of later epochs are queued as soon as their gathers finish, so the DMA
-    // engine never waits for this thread to wake up
+    // the copier thread queues the copies; wait for them to finish
+    std::unique_lock<std::recursive_mutex> lk(b->mu);
     for (;;) {
-      if (issue_ready_copies(s, b))
+      if (!b->copy_error.empty()) {
+        set_error(b->copy_error);
         return 1;
+      }
       if (t.copied && (!t.mask_a || t.copied_a)) {
         hipError_t q = hipEventQuery(t.copied);
         if (q == hipSuccess && t.mask_a)
@@ -392,15 +422,12 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
       }
       if (!block)
         return 2;
+      lk.unlock();
       std::this_thread::yield();
+      lk.lock();
     }
-    b->event_pool.push_back(t.copied);
-    t.copied = nullptr;
-    if (t.mask_a) {
-      b->event_pool.push_back(t.copied_a);
-      b->event_pool.push_back(t.done_a);
-      t.copied_a = t.done_a = nullptr;
-    }
+    // (the ticket keeps its copy events until it leaves the queue: the
+    // copier thread takes a ticket without them for one still to copy)
   } else if (block) {
     BCHECK(hipEventSynchronize(t.done));
   } else {
@@ -415,10 +442,13 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     if (hipEventElapsedTime(&ms, t.c0, t.c1) == hipSuccess)
       metric_add(s->metrics.compress, ms, (double)s->plan.total_chunks * s->chunk_bytes, 0);
   }
-  if (t.c0)
-    b->timing_pool.push_back(t.c0);
-  if (t.c1)
-    b->timing_pool.push_back(t.c1);
+  {
+    std::lock_guard<std::recursive_mutex> lk(b->mu);
+    if (t.c0)
+      b->timing_pool.push_back(t.c0);
+    if (t.c1)
+      b->timing_pool.push_back(t.c1);
+  }
   scatter_collect_ready(s);
   const auto t1 = std::chrono::steady_clock::now();
   for (int lv = 0; lv < s->plan.nlod; ++lv) {
@@ -438,7 +468,11 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
   s->metrics.sink.count++;
   s->latencies.push_back(std::chrono::duration<double>(t2 - t.t0).count());
   s->epochs_written++;
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
   b->event_pool.push_back(t.done);
+  for (hipEvent_t e : { t.copied, t.copied_a, t.done_a })
+    if (e)
+      b->event_pool.push_back(e);
   b->inflight.pop_front();
   return 0;
 }
@@ -536,6 +570,8 @@ int batched_init(tile_stream_gpu* s)
         capture(s->st_comp, &P.compress_exec, [&](hipStream_t st) { return launch_compress(s, ps, st); }))
       return 1;
   }
+  if (b->dma)
+    b->copier = std::thread(copier_loop, s, b);
   return 0;
 }
 
@@ -544,6 +580,14 @@ void batched_free(tile_stream_gpu* s)
   BatchedState* b = s->batched;
   if (!b)
     return;
+  if (b->copier.joinable()) {
+    {
+      std::lock_guard<std::recursive_mutex> lk(b->mu);
+      b->stop = true;
+    }
+    b->cv.notify_all();
+    b->copier.join();
+  }
   if (b->st_agg)
     (void)hipStreamSynchronize(b->st_agg);
   for (auto z : b->st_zip)
@@ -567,6 +611,10 @@ void batched_free(tile_stream_gpu* s)
     (void)hipEventDestroy(t.done);
     if (t.copied)
       (void)hipEventDestroy(t.copied);
+    if (t.done_a)
+      (void)hipEventDestroy(t.done_a);
+    if (t.copied_a)
+      (void)hipEventDestroy(t.copied_a);
     if (t.c0)
       (void)hipEventDestroy(t.c0);
     if (t.c1)
@@ -644,7 +692,11 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
                             std::chrono::steady_clock::now() };
     t.mask_a = 1u;
     t.done_a = done_a;
-    b->inflight.push_back(t);
+    {
+      std::lock_guard<std::recursive_mutex> lk(b->mu);
+      b->inflight.push_back(t);
+    }
+    b->cv.notify_all();
     for (;;) {
       const int r = deliver_one(s, b, false);
       if (r == 1)
@@ -704,8 +756,12 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     return 1;
   }
   BCHECK(hipEventRecord(done, as));
-  b->inflight.push_back({ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
-                          std::chrono::steady_clock::now() });
+  {
+    std::lock_guard<std::recursive_mutex> lk(b->mu);
+    b->inflight.push_back({ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
+                            std::chrono::steady_clock::now() });
+  }
+  b->cv.notify_all();
   // opportunistic in-order delivery of whatever has finished
   for (;;) {
     const int r = deliver_one(s, b, false);
