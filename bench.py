@@ -158,7 +158,8 @@ def run(name, frames, steps, warmup, pipelines, host_input, options=None):
 LOD_PASSES = {"lod_gather": "tiles_k (linear epoch -> level-0 Morton)",
               "lod_reduce": "pyramid_k (every level from level 0, one launch)",
               "lod_append_fold": "fold_emit_k (append-dim fold and emit, all levels, one launch)",
-              "lod_morton_chunk": "tiles_k (Morton levels -> chunk pool, all levels)"}
+              "lod_morton_chunk": "tiles_levels_k (Morton levels -> chunk pool, levels of one "
+                                  "chunk geometry in one launch)"}
 
 
 def snapshot(s):
