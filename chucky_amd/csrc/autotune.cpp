@@ -43,14 +43,19 @@ uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch)
   return kFloor;
 }
 
-uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log)
+uint64_t lz4_wave_slots(int hash_log)
 {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // one LZ4 wavefront holds a (4 << hash_log)-byte LDS table
   const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / (4ull << hash_log));
-  const uint64_t slots = (uint64_t)cus * std::min<uint64_t>(per_cu, 16);
+  return (uint64_t)cus * std::min<uint64_t>(per_cu, 16);
+}
+
+uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log)
+{
+  const uint64_t slots = lz4_wave_slots(hash_log);
   if (chunks_per_launch >= slots)
     return 0;
   constexpr uint32_t kMinSeg = 16 * 1024;

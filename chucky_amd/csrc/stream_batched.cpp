@@ -64,6 +64,9 @@ struct BatchedState
   bool graphs = false;
   std::vector<hipStream_t> st_zip;
   uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
+  // LZ4 segment scratch, one slice of lz4_stride bytes per compression queue
+  uint8_t* d_lz4 = nullptr;
+  uint64_t lz4_stride = 0;
   bool dma = true; // gathered bytes reach host memory by DMA copies
   uint32_t R = 0; // host slots
   struct HostSlot
@@ -194,7 +197,7 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
     BCHECK(launch_lz4(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                       s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
                       s->d_seg_sizes + (uint64_t)slot * T * s->nseg, s->lz4_hash_log, s->lz4_seg,
-                      s->d_lz4_scratch, st));
+                      b->d_lz4 ? b->d_lz4 + z * b->lz4_stride : s->d_lz4_scratch, st));
   else if (codec.id == CODEC_ZSTD)
     BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
                        s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
@@ -493,6 +496,14 @@ int batched_init(tile_stream_gpu* s)
     const uint64_t blocks = std::max<uint64_t>((uint64_t)p.total_chunks * s->nseg, 1);
     b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, kMaxZip), std::max<uint64_t>(1, s->zworkers / blocks));
     b->zip_workers = (uint32_t)std::min<uint64_t>(blocks, s->zworkers / b->NZ);
+  } else if (p.config.codec.id == CODEC_LZ4_NON_STANDARD) {
+    // LZ4 launches one wavefront per chunk or segment; epochs whose
+    // wavefronts fill at most half of the device compress side by side
+    const uint64_t per_chunk = s->lz4_seg ? (s->chunk_bytes + s->lz4_seg - 1) / s->lz4_seg : 1;
+    const uint64_t waves = std::max<uint64_t>((uint64_t)p.total_chunks * per_chunk, 1);
+    b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, kMaxZip),
+                                         std::max<uint64_t>(1, lz4_wave_slots(s->lz4_hash_log) / waves));
+    b->lz4_stride = align_up(lz4_scratch_bytes(s->chunk_bytes, p.total_chunks, s->lz4_seg), 256);
   }
   // HIP maps streams onto its hardware queues (GPU_MAX_HW_QUEUES, 4 by
   // default) in creation order, and kernels of streams sharing a queue run
@@ -510,6 +521,8 @@ int batched_init(tile_stream_gpu* s)
   }
   BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
+    return 1;
+  if (b->NZ > 1 && b->lz4_stride && dev_alloc(s, &b->d_lz4, b->NZ * b->lz4_stride))
     return 1;
   // the compressors' work counters start at zero and each launch leaves them so
   BCHECK(hipMemsetAsync(b->d_queues, 0, 16 * b->NZ * sizeof(uint32_t), s->st_comp));

@@ -157,6 +157,8 @@ uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch);
 uint32_t tune_host_gather_workgroups();
 bool tune_zstd_chunk_tables(uint32_t block);
 uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log);
+// LZ4 wavefronts resident at once on the device (LDS-bound by the hash table)
+uint64_t lz4_wave_slots(int hash_log);
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
 int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);

@@ -174,6 +174,10 @@ BATCHED_CASES = {
     "volume_lz4_k2": (lambda: xor_volume((64, 64, 64)),
                       [Dim(64, 16, 2), Dim(64, 16, 2), Dim(64, 16, 4)], "u16", "lz4",
                       {"epochs_per_batch": 2}),
+    # 256 KiB chunks, 4 per epoch: 16 KiB segments, two epochs compress side by side
+    "volume_lz4_seg": (lambda: xor_volume((128, 128, 128)),
+                       [Dim(128, 32, 2), Dim(128, 64, 1), Dim(128, 64, 1)], "u16", "lz4",
+                       {"epochs_per_batch": 2}),
     "partial_none": (lambda: random_typed(20 * 16 * 24, "u8", seed=11),
                      [Dim(20, 8, 2), Dim(16, 8, 1), Dim(24, 8, 3)], "u8", "none", {}),
     "lod_inner": (lambda: random_typed(16 * 48 * 40, "u16", seed=21),
