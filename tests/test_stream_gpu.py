@@ -174,7 +174,7 @@ BATCHED_CASES = {
     "volume_lz4_k2": (lambda: xor_volume((64, 64, 64)),
                       [Dim(64, 16, 2), Dim(64, 16, 2), Dim(64, 16, 4)], "u16", "lz4",
                       {"epochs_per_batch": 2}),
-    # 256 KiB chunks, 4 per epoch: 16 KiB segments, two epochs compress side by side
+    # 256 KiB chunks, 4 per epoch: 8 KiB segments, two epochs compress side by side
     "volume_lz4_seg": (lambda: xor_volume((128, 128, 128)),
                        [Dim(128, 32, 2), Dim(128, 64, 1), Dim(128, 64, 1)], "u16", "lz4",
                        {"epochs_per_batch": 2}),
@@ -222,7 +222,7 @@ LZ4_SEGMENT_DATA = {
 @pytest.mark.parametrize("name", sorted(LZ4_SEGMENT_DATA))
 @pytest.mark.parametrize("pipeline", [0, 1])
 def test_lz4_multi_wavefront_chunks(name, pipeline):
-    # 128 KiB chunks, few per launch: each chunk is parsed in 16 KiB segments
+    # 128 KiB chunks, few per launch: each chunk is parsed in 8 KiB segments
     # by separate wavefronts and stitched into one LZ4 block
     cfg = ca.Config([Dim(64, 32, 2), Dim(64, 32, 2), Dim(64, 64, 1)], dtype="u16", codec="lz4")
     _compare(cfg, LZ4_SEGMENT_DATA[name](), pieces=2, pipeline=pipeline)
