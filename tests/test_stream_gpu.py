@@ -183,6 +183,12 @@ BATCHED_CASES = {
     "lod_inner": (lambda: random_typed(16 * 48 * 40, "u16", seed=21),
                   [Dim(16, 8, 1), Dim(48, 8, 0, downsample=True), Dim(40, 8, 0, downsample=True)],
                   "u16", "zstd", {}),
+    # 32 KiB chunks of several blocks, 3 levels emitted every epoch: with direct
+    # launches each epoch is split (coarse chunks compress beside level 0)
+    "lod_zstd_split": (lambda: np.clip(np.random.default_rng(6).poisson(100, 64 * 128 * 128), 0, 4095)
+                       .astype(np.uint16),
+                       [Dim(64, 16, 2), Dim(128, 32, 0, downsample=True), Dim(128, 32, 0, downsample=True)],
+                       "u16", "zstd", {}),
     "lod_dim0": (lambda: random_typed(40 * 24 * 20, "f32", seed=33),
                  [Dim(40, 4, 2, downsample=True), Dim(24, 4, 0, downsample=True),
                   Dim(20, 4, 0, downsample=True)], "f32", "none", {"epochs_per_batch": 4}),
