@@ -77,6 +77,15 @@ Writer._fields_ = [("append", C.CFUNCTYPE(WriterResult, C.POINTER(Writer), Slice
                    ("flush", C.CFUNCTYPE(WriterResult, C.POINTER(Writer)))]
 
 
+class MultiarrayWriter(C.Structure):
+    pass
+
+
+MultiarrayWriter._fields_ = [
+    ("update", C.CFUNCTYPE(WriterResult, C.POINTER(MultiarrayWriter), C.c_int, Slice)),
+    ("flush", C.CFUNCTYPE(WriterResult, C.POINTER(MultiarrayWriter)))]
+
+
 class StreamMetric(C.Structure):
     _fields_ = [("name", C.c_char_p), ("ms", C.c_float), ("best_ms", C.c_float),
                 ("input_bytes", C.c_double), ("output_bytes", C.c_double),
@@ -149,6 +158,11 @@ _sig("tile_stream_gpu_epoch_latencies", C.c_size_t, _P, C.POINTER(C.c_double), C
 _sig("tile_stream_gpu_level_geometry", C.c_int, _P, C.c_int, C.POINTER(C.c_uint64),
      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
 _sig("tile_stream_gpu_options_default", None, C.POINTER(StreamOptions))
+_sig("multiarray_tile_stream_gpu_create", _P, C.c_int, C.POINTER(TileStreamConfiguration),
+     C.POINTER(C.c_void_p), C.c_int)
+_sig("multiarray_tile_stream_gpu_destroy", None, _P)
+_sig("multiarray_tile_stream_gpu_writer", C.POINTER(MultiarrayWriter), _P)
+_sig("multiarray_tile_stream_gpu_get_metrics", StreamMetrics, _P)
 _sig("writer_append", WriterResult, C.POINTER(Writer), Slice)
 _sig("writer_flush", WriterResult, C.POINTER(Writer))
 _sig("chucky_last_error", C.c_char_p)
@@ -235,8 +249,8 @@ class DeviceBuffer:
     def __del__(self):
         self.free()
 
-from .api import (Config, GpuStream, MemSink, DiscardSink, last_error,  # noqa: E402
+from .api import (Config, GpuStream, GpuMultiArray, MemSink, DiscardSink, last_error,  # noqa: E402
                   crc32c, memory_estimate)
 
-__all__ = ["lib", "PIPELINE_PLAIN", "PIPELINE_BATCHED", "DeviceBuffer", "Config", "GpuStream", "MemSink", "DiscardSink", "last_error", "crc32c",
+__all__ = ["lib", "PIPELINE_PLAIN", "PIPELINE_BATCHED", "DeviceBuffer", "Config", "GpuStream", "GpuMultiArray", "MemSink", "DiscardSink", "last_error", "crc32c",
            "memory_estimate", "DTYPES", "CODEC_ID", "REDUCE_ID"]

@@ -293,3 +293,44 @@ class GpuStream:
 
     def __del__(self):
         self.close()
+
+
+class GpuMultiArray:
+    """multiarray_tile_stream_gpu_* through its multiarray_writer vtable: one
+    writer, several arrays, each with its own GPU tile stream."""
+
+    def __init__(self, cfgs, sinks):
+        self.cfgs = list(cfgs)
+        self.sinks = list(sinks)
+        arr = (TileStreamConfiguration * len(self.cfgs))()
+        self._c = [c.to_c() for c in self.cfgs]
+        for i, c in enumerate(self._c):
+            arr[i] = c
+        self._arr = arr
+        ss = (C.c_void_p * len(self.sinks))(*[s.shard_sink for s in self.sinks])
+        self.h = lib.multiarray_tile_stream_gpu_create(len(self.cfgs), arr, ss, 1)
+        if not self.h:
+            raise RuntimeError("multiarray_tile_stream_gpu_create failed: " + last_error())
+        self.w = lib.multiarray_tile_stream_gpu_writer(self.h)
+
+    def update(self, index: int, data):
+        """Returns (error code, bytes consumed)."""
+        buf = np.ascontiguousarray(data)
+        beg = buf.ctypes.data
+        r = self.w.contents.update(self.w, index, Slice(beg, beg + buf.nbytes))
+        rest = (r.rest.end or 0) - (r.rest.beg or 0) if r.rest.beg else 0
+        return r.error, buf.nbytes - rest
+
+    def flush(self) -> int:
+        return self.w.contents.flush(self.w).error
+
+    def metrics(self):
+        return lib.multiarray_tile_stream_gpu_get_metrics(self.h)
+
+    def close(self):
+        if self.h:
+            lib.multiarray_tile_stream_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
