@@ -37,7 +37,9 @@ def test_headers_declare_the_drop_in_api():
                "tile_stream_gpu_advise_chunk_sizes", "tile_stream_gpu_get_metrics",
                "tile_stream_gpu_layout", "tile_stream_gpu_writer", "tile_stream_gpu_cursor",
                "tile_stream_gpu_status", "writer_append", "writer_flush", "writer_append_wait",
-               "dims_create", "dims_n_append", "dims_validate", "zarr_fs_sink_create"):
+               "dims_create", "dims_n_append", "dims_validate", "zarr_fs_sink_create",
+               "multiarray_tile_stream_gpu_create", "multiarray_tile_stream_gpu_destroy",
+               "multiarray_tile_stream_gpu_writer", "multiarray_tile_stream_gpu_get_metrics"):
         assert fn in names, fn
 
 
