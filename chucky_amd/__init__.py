@@ -176,6 +176,8 @@ _sig("chucky_mem_sink_get", C.c_int, _P, C.c_size_t, C.POINTER(C.c_uint8),
 _sig("chucky_mem_sink_append_sizes", C.c_int, _P, C.c_uint8, C.POINTER(C.c_uint64),
      C.POINTER(C.c_uint8))
 _sig("chucky_mem_sink_inject_failure", None, _P, C.c_int, C.c_int)
+_sig("chucky_mem_sink_set_deferred", None, _P, C.c_int)
+_sig("chucky_mem_sink_drain", None, _P)
 _sig("chucky_discard_sink_create", _P)
 _sig("chucky_discard_sink_destroy", None, _P)
 _sig("chucky_discard_sink_as_shard_sink", _P, _P)

@@ -87,14 +87,17 @@ class Config:
 class MemSink:
     """In-memory shard sink (chucky_mem_sink_*)."""
 
-    def __init__(self):
+    def __init__(self, deferred: bool = False):
         self.h = lib.chucky_mem_sink_create()
+        if deferred:
+            lib.chucky_mem_sink_set_deferred(self.h, 1)
 
     @property
     def shard_sink(self):
         return lib.chucky_mem_sink_as_shard_sink(self.h)
 
     def shards(self) -> Dict[Tuple[int, int], bytes]:
+        lib.chucky_mem_sink_drain(self.h)
         out = {}
         n = lib.chucky_mem_sink_count(self.h)
         for i in range(n):

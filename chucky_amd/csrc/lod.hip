@@ -48,6 +48,18 @@ template<> struct Acc<_Float16> { using type = float; };
 
 template<typename T> constexpr bool is_fp = std::is_floating_point<T>::value || std::is_same<T, _Float16>::value;
 
+// Float mean emit: the accumulated sum over `count` epochs, rounded once to
+// T. Halfs divide in float (correctly rounded) and convert to half, which is
+// the correctly rounded half quotient (float carries more than 2x11+2 bits).
+template<typename T>
+__device__ __forceinline__ T div_count(T a, uint32_t count)
+{
+  if constexpr (std::is_same<T, _Float16>::value)
+    return (T)((float)a / (float)count);
+  else
+    return (T)(a / (T)count);
+}
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 struct TileGeom
@@ -594,7 +606,7 @@ __global__ __launch_bounds__(256) void emit_k(T* __restrict__ dst, const T* __re
     return;
   if constexpr (is_fp<T>) {
     if (method == lod_reduce_mean) {
-      dst[g] = (T)(accum[g] / (T)count);
+      dst[g] = div_count<T>(accum[g], count);
       return;
     }
   }
@@ -647,7 +659,7 @@ __global__ __launch_bounds__(256) void fold_emit_k(FoldLevels<T> f, int method)
     T out = a;
     if constexpr (is_fp<T>) {
       if (method == lod_reduce_mean)
-        out = (T)(a / (T)f.emit[k]);
+        out = div_count<T>(a, f.emit[k]);
     }
     f.x[k][i] = out;
   }

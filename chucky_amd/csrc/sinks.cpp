@@ -25,6 +25,18 @@ struct MemWriter
 
 } // namespace
 
+// A zero-copy write held back until its level's fence is waited for (or the
+// sink is drained): the caller's buffer is read only then, as an
+// asynchronous I/O queue would.
+struct PendingWrite
+{
+  MemWriter* w;
+  uint64_t offset;
+  const uint8_t* src;
+  size_t n;
+  uint64_t seq;
+};
+
 struct chucky_mem_sink
 {
   shard_sink base;
@@ -33,11 +45,22 @@ struct chucky_mem_sink
   std::map<std::pair<uint8_t, uint64_t>, MemWriter*> by_key;
   std::map<uint8_t, std::vector<uint64_t>> append_sizes;
   int fail_open = 0, fail_write = 0;
+  int deferred = 0;
+  std::map<uint8_t, uint64_t> level_seq;             // zero-copy writes issued per level
+  std::map<uint8_t, std::deque<PendingWrite>> pending; // not yet copied, in issue order
 };
 
 namespace {
 
 chucky_mem_sink* mem_of(shard_sink* s) { return reinterpret_cast<chucky_mem_sink*>(s); }
+
+void store(MemWriter* w, uint64_t offset, const void* beg, size_t n)
+{
+  if (w->bytes.size() < offset + n)
+    w->bytes.resize(offset + n);
+  if (n)
+    std::memcpy(w->bytes.data() + offset, beg, n);
+}
 
 int mem_write(shard_writer* self, uint64_t offset, const void* beg, const void* end)
 {
@@ -46,12 +69,45 @@ int mem_write(shard_writer* self, uint64_t offset, const void* beg, const void* 
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->fail_write)
     return 1;
-  const size_t n = (const uint8_t*)end - (const uint8_t*)beg;
-  if (w->bytes.size() < offset + n)
-    w->bytes.resize(offset + n);
-  if (n)
-    std::memcpy(w->bytes.data() + offset, beg, n);
+  store(w, offset, beg, (const uint8_t*)end - (const uint8_t*)beg);
   return 0;
+}
+
+int mem_write_direct(shard_writer* self, uint64_t offset, const void* beg, const void* end)
+{
+  MemWriter* w = reinterpret_cast<MemWriter*>(self);
+  chucky_mem_sink* s = reinterpret_cast<chucky_mem_sink*>(w->owner);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->fail_write)
+    return 1;
+  const size_t n = (const uint8_t*)end - (const uint8_t*)beg;
+  s->pending[w->level].push_back({ w, offset, (const uint8_t*)beg, n, ++s->level_seq[w->level] });
+  return 0;
+}
+
+// Copies the level's held-back writes with sequence <= upto (mu held).
+void apply_pending(chucky_mem_sink* s, uint8_t level, uint64_t upto)
+{
+  auto& q = s->pending[level];
+  while (!q.empty() && q.front().seq <= upto) {
+    const PendingWrite& p = q.front();
+    store(p.w, p.offset, p.src, p.n);
+    q.pop_front();
+  }
+}
+
+io_event mem_record_fence(shard_sink* self, uint8_t level)
+{
+  chucky_mem_sink* s = reinterpret_cast<chucky_mem_sink*>(self);
+  std::lock_guard<std::mutex> lk(s->mu);
+  return io_event{ s->level_seq[level] };
+}
+
+void mem_wait_fence(shard_sink* self, uint8_t level, io_event ev)
+{
+  chucky_mem_sink* s = reinterpret_cast<chucky_mem_sink*>(self);
+  std::lock_guard<std::mutex> lk(s->mu);
+  apply_pending(s, level, ev.seq);
 }
 
 int mem_finalize(shard_writer* self)
@@ -78,7 +134,7 @@ shard_writer* mem_open(shard_sink* self, uint8_t level, uint64_t shard_index)
   s->writers.emplace_back();
   MemWriter& w = s->writers.back();
   w.base.write = mem_write;
-  w.base.write_direct = nullptr;
+  w.base.write_direct = s->deferred ? mem_write_direct : nullptr;
   w.base.finalize = mem_finalize;
   w.owner = reinterpret_cast<struct chucky_mem_sink_impl*>(s);
   w.level = level;
@@ -151,6 +207,23 @@ void chucky_mem_sink_inject_failure(struct chucky_mem_sink* s, int fail_open, in
   std::lock_guard<std::mutex> lk(s->mu);
   s->fail_open = fail_open;
   s->fail_write = fail_write;
+}
+
+void chucky_mem_sink_set_deferred(struct chucky_mem_sink* s, int deferred)
+{
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->deferred = deferred;
+  s->base.record_fence = deferred ? mem_record_fence : nullptr;
+  s->base.wait_fence = deferred ? mem_wait_fence : nullptr;
+  for (auto& w : s->writers)
+    w.base.write_direct = deferred ? mem_write_direct : nullptr;
+}
+
+void chucky_mem_sink_drain(struct chucky_mem_sink* s)
+{
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto& kv : s->pending)
+    apply_pending(s, kv.first, ~0ull);
 }
 
 } // extern "C"

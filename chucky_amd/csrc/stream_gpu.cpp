@@ -204,8 +204,12 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
   HIPCHECK(launch_aggregate_gather(src, s->d_gather, s->d_perm, n_items, s->d_offsets, s->d_agg,
                                    s->st_comp));
   HIPCHECK(hipEventRecord(e1, s->st_comp));
+  // Every level's zero-copy writes read h_agg until their fence retires
+  // (write_direct contract), so all levels' fences must pass before the
+  // copy overwrites it.
   if (s->sink->wait_fence)
-    s->sink->wait_fence(s->sink, (uint8_t)lv, s->io_done[lv]);
+    for (int j = 0; j < s->plan.nlod; ++j)
+      s->sink->wait_fence(s->sink, (uint8_t)j, s->io_done[j]);
   HIPCHECK(hipMemcpyAsync(s->h_agg, s->d_agg, copy, hipMemcpyDeviceToHost, s->st_comp));
   HIPCHECK(hipEventRecord(e2, s->st_comp));
   HIPCHECK(hipEventSynchronize(e2));

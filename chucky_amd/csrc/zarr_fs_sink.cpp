@@ -560,7 +560,9 @@ shard_writer* fs_open(shard_sink* self, uint8_t, uint64_t shard_index)
   for (int d = 0; d < zs->rank; ++d)
     sc[d] = zs->ext[d].shard_count;
   const std::string path = zs->array_dir + "/" + shard_key(zs->rank, sc.data(), shard_index);
-  const int flags = O_WRONLY | O_CREAT | (zs->unbuffered ? O_DIRECT : 0);
+  // a shard file starts empty: an older, longer file at the same key would
+  // keep its tail (and its stale index) past the new index
+  const int flags = O_WRONLY | O_CREAT | O_TRUNC | (zs->unbuffered ? O_DIRECT : 0);
   int fd = ::open(path.c_str(), flags, 0644);
   if (fd < 0) {
     const size_t slash = path.rfind('/');

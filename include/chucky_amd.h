@@ -129,6 +129,13 @@ int chucky_mem_sink_append_sizes(const struct chucky_mem_sink* s, uint8_t level,
                                  uint64_t* out, uint8_t* n);
 /* Make open() fail (fail_open) or write() fail (fail_write) from now on. */
 void chucky_mem_sink_inject_failure(struct chucky_mem_sink* s, int fail_open, int fail_write);
+/* Deferred mode: zero-copy writes (write_direct) are held and their bytes
+ * read from the caller's buffer only when the level's fence is waited for or
+ * the sink is drained, as an asynchronous I/O queue reads them; fences are
+ * counted per level. Exposes buffers reused before their fence retires. */
+void chucky_mem_sink_set_deferred(struct chucky_mem_sink* s, int deferred);
+/* Copies every held-back write (call before reading shards in deferred mode). */
+void chucky_mem_sink_drain(struct chucky_mem_sink* s);
 
 struct chucky_discard_sink;
 struct chucky_discard_sink* chucky_discard_sink_create(void);

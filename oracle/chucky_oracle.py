@@ -125,6 +125,89 @@ def z_increment(coords, p):
     return c
 
 
+def _ceil_log2_array(v):
+    """ceil_log2 elementwise: the least p with 2^p >= v (integer steps only)."""
+    v = np.asarray(v, dtype=np.int64)
+    out = np.zeros(v.shape, dtype=np.int64)
+    while True:
+        short = (np.int64(1) << out) < v
+        if not short.any():
+            return out
+        out += short
+
+
+def _morton_rank_fixed_p(shape, c, p, depth):
+    nd = len(shape)
+    levels = p + depth
+    n = c.shape[1]
+    rank = np.zeros(n, dtype=np.int64)
+    prefix = np.zeros((nd, n), dtype=np.int64)
+    for lv in range(levels):
+        scale = 1 << (levels - 1 - lv)
+        bits = [((c[d] >> (p - 1 - lv)) & 1) if lv < p else np.zeros(n, dtype=np.int64)
+                for d in range(nd)]
+
+        def inside(ext, lo):
+            return np.where(lo >= ext, 0, np.minimum(ext - lo, scale))
+        lo_h = [inside(shape[d], prefix[d] * 2 * scale) for d in range(nd)]
+        hi_h = [inside(shape[d], (prefix[d] * 2 + 1) * scale) for d in range(nd)]
+        below = [np.ones(n, dtype=np.int64)]
+        for d in range(nd):
+            below.append(below[d] * (lo_h[d] + hi_h[d]))
+        fixed = np.ones(n, dtype=np.int64)
+        for d in range(nd - 1, -1, -1):
+            hi = bits[d] == 1
+            rank += np.where(hi, fixed * lo_h[d] * below[d], 0)
+            fixed = fixed * np.where(hi, hi_h[d], lo_h[d])
+        for d in range(nd):
+            prefix[d] = prefix[d] * 2 + bits[d]
+    return rank
+
+
+def morton_rank_array(shape, coords, depth=0):
+    """morton_rank over arrays: coords is (len(shape), N); the same per-element
+    arithmetic, elements grouped by their bit depth p."""
+    c = np.asarray(coords, dtype=np.int64).reshape(len(shape), -1)
+    base = ceil_log2(max(shape)) if len(shape) else 0
+    p_el = np.maximum(base, _ceil_log2_array(c.max(axis=0) + 1)) if c.size else np.zeros(0, np.int64)
+    out = np.zeros(c.shape[1], dtype=np.int64)
+    for p in np.unique(p_el):
+        sel = p_el == p
+        out[sel] = _morton_rank_fixed_p(shape, c[:, sel], int(p), depth)
+    return out
+
+
+def z_increment_array(coords, p):
+    """z_increment over arrays (coords: (nd, N))."""
+    c = np.array(coords, dtype=np.int64).reshape(len(coords), -1)
+    done = np.zeros(c.shape[1], dtype=bool)
+    for bit in range(p):
+        m = 1 << bit
+        for d in range(c.shape[0]):
+            act = ~done
+            c[d, act] ^= m
+            done |= act & ((c[d] & m) != 0)
+    rest = ~done
+    c[:, rest] = 0
+    c[0, rest] = 1 << p
+    return c
+
+
+def window_ends_array(child_shape, parent_shape):
+    """window_ends computed over all parents at once."""
+    n_par = int(np.prod(parent_shape))
+    j = np.arange(n_par, dtype=np.int64)
+    c, r = [], j
+    for s in parent_shape:
+        c.append(r % s)
+        r = r // s
+    c = np.stack(c)
+    zp = ceil_log2(max(parent_shape))
+    ends = np.zeros(n_par, dtype=np.int64)
+    ends[morton_rank_array(parent_shape, c, 0)] = morton_rank_array(child_shape, z_increment_array(c, zp), 1)
+    return ends
+
+
 def window_ends(child_shape, parent_shape):
     """ends[morton_rank(parent)] = child rank of the next parent's first child."""
     n_par = int(np.prod(parent_shape))
@@ -434,6 +517,16 @@ def add_shift(a, b, s, dt):
     return r.astype(dt)
 
 
+def emit_accum(acc, count, append_method):
+    """Accumulator emit (src/cpu/lod.cpp:545-602; f16: src/gpu/lod.cu:827-845):
+    a float mean divides the running sum by the epoch count, everything else
+    is already final."""
+    dt = acc.dtype
+    if dt.kind == "f" and append_method == MEAN:
+        return (acc / dt.type(count)).astype(dt)
+    return acc.copy()
+
+
 class LodState:
     def __init__(self, plan: Plan, dtype):
         self.p = plan
@@ -444,51 +537,42 @@ class LodState:
         for d in range(p.rank - 2, -1, -1):
             fstride[d] = fstride[d + 1] * full[d + 1]
         self.fstride = fstride
+        # the LUT loops of src/cpu/lod.cpp:182-268 (gather) and 298-359
+        # (chunk scatter), evaluated over every cell at once: row-major
+        # coordinates (last dim fastest), Morton rank -> offset
         lshape = [p.levels[0].shape[d] for d in p.lod_dims]
         n0 = p.levels[0].lod_nelem
+        c = np.unravel_index(np.arange(n0, dtype=np.int64), lshape)
+        off = sum(c[k].astype(np.int64) * fstride[p.lod_dims[k]] for k in range(len(lshape)))
         self.gather_lut = np.zeros(n0, dtype=np.int64)
-        for g in range(n0):
-            c, r, off = [0] * len(lshape), g, 0
-            for k in range(len(lshape) - 1, -1, -1):
-                c[k] = r % lshape[k]
-                r //= lshape[k]
-                off += c[k] * fstride[p.lod_dims[k]]
-            self.gather_lut[morton_rank(lshape, c)] = off
+        self.gather_lut[morton_rank_array(lshape, np.stack(c))] = off
+        fc = np.unravel_index(np.arange(p.fixed_count, dtype=np.int64), p.fixed_shape) if p.fixed_shape else ()
         self.gather_fixed = np.zeros(p.fixed_count, dtype=np.int64)
-        for b in range(p.fixed_count):
-            r, off = b, 0
-            for k in range(len(p.fixed_shape) - 1, -1, -1):
-                off += (r % p.fixed_shape[k]) * fstride[p.fixed_dims[k]]
-                r //= p.fixed_shape[k]
-            self.gather_fixed[b] = off
+        for k in range(len(p.fixed_shape)):
+            self.gather_fixed += fc[k].astype(np.int64) * fstride[p.fixed_dims[k]]
         self.ends = []
         for l in range(p.nlod - 1):
             ch = [p.levels[l].shape[d] for d in p.lod_dims]
             pa = [p.levels[l + 1].shape[d] for d in p.lod_dims]
-            self.ends.append(window_ends(ch, pa))
+            self.ends.append(window_ends_array(ch, pa))
         self.chunk_lut, self.chunk_fixed = [], []
         for lv, L in enumerate(p.levels):
             ls = [L.shape[d] for d in p.lod_dims]
+            c = np.unravel_index(np.arange(L.lod_nelem, dtype=np.int64), ls)
+            off = np.zeros(L.lod_nelem, dtype=np.int64)
+            for k in range(len(ls)):
+                d = p.lod_dims[k]
+                cs = L.lifted_shape[2 * d + 1]
+                ck = c[k].astype(np.int64)
+                off += (ck // cs) * L.lifted_strides[2 * d] + (ck % cs) * L.lifted_strides[2 * d + 1]
             lut = np.zeros(L.lod_nelem, dtype=np.int64)
-            for g in range(L.lod_nelem):
-                c, r, off = [0] * len(ls), g, 0
-                for k in range(len(ls) - 1, -1, -1):
-                    c[k] = r % ls[k]
-                    r //= ls[k]
-                    d = p.lod_dims[k]
-                    cs = L.lifted_shape[2 * d + 1]
-                    off += (c[k] // cs) * L.lifted_strides[2 * d] + (c[k] % cs) * L.lifted_strides[2 * d + 1]
-                lut[morton_rank(ls, c)] = off
-            fx = np.zeros(p.fixed_count, dtype=np.int64)
-            for b in range(p.fixed_count):
-                r, off = b, 0
-                for k in range(len(p.fixed_shape) - 1, -1, -1):
-                    coord = r % p.fixed_shape[k]
-                    r //= p.fixed_shape[k]
-                    d = p.fixed_dims[k]
-                    cs = L.lifted_shape[2 * d + 1]
-                    off += (coord // cs) * L.lifted_strides[2 * d] + (coord % cs) * L.lifted_strides[2 * d + 1]
-                fx[b] = off + L.chunk_offset * L.chunk_elements
+            lut[morton_rank_array(ls, np.stack(c))] = off
+            fx = np.full(p.fixed_count, L.chunk_offset * L.chunk_elements, dtype=np.int64)
+            for k in range(len(p.fixed_shape)):
+                d = p.fixed_dims[k]
+                cs = L.lifted_shape[2 * d + 1]
+                ck = fc[k].astype(np.int64)
+                fx += (ck // cs) * L.lifted_strides[2 * d] + (ck % cs) * L.lifted_strides[2 * d + 1]
             self.chunk_lut.append(lut)
             self.chunk_fixed.append(fx)
         self.accum = [None] * p.nlod
@@ -505,6 +589,31 @@ class LodState:
             vals.append(out.reshape(-1))
         del n0
         return vals
+
+    def fold_emit(self, vals, append_method: int) -> int:
+        """Append-dim fold of one epoch's levels >= 1 into the accumulators
+        (src/cpu/lod.cpp:363-438) and the emit of every level whose window of
+        2^level epochs completes (src/cpu/pipeline.c:258-288); emitted values
+        replace vals[level]. Returns the active-level mask."""
+        p, dt = self.p, self.dt
+        for lv in range(1, p.nlod):
+            v = vals[lv]
+            if self.counts[lv] == 0:
+                self.accum[lv] = v.copy()
+            elif append_method == MEAN:
+                self.accum[lv] = add_shift(self.accum[lv], v, lv, dt)
+            elif append_method == MIN:
+                self.accum[lv] = np.where(v < self.accum[lv], v, self.accum[lv])
+            else:
+                self.accum[lv] = np.where(v > self.accum[lv], v, self.accum[lv])
+        mask = 1
+        for lv in range(1, p.nlod):
+            self.counts[lv] += 1
+            if self.counts[lv] >= (1 << lv):
+                vals[lv] = emit_accum(self.accum[lv], self.counts[lv], append_method)
+                self.counts[lv] = 0
+                mask |= 1 << lv
+        return mask
 
     def scatter(self, pool_epoch: np.ndarray, lv: int, vals: np.ndarray):
         idx = (self.chunk_fixed[lv][:, None] + self.chunk_lut[lv][None, :]).reshape(-1)
@@ -608,32 +717,14 @@ def write_stream(dims: List[ODim], dtype, data: np.ndarray, reduce_method=MEAN,
         vals = lod.reduce_levels(linear, reduce_method)
         mask = (1 << plan.nlod) - 1
         if plan.append_downsample:
-            mask = 1
-            for lv in range(1, plan.nlod):
-                v = vals[lv]
-                if lod.counts[lv] == 0:
-                    lod.accum[lv] = v.copy()
-                elif append_method == MEAN:
-                    lod.accum[lv] = add_shift(lod.accum[lv], v, lv, dt)
-                elif append_method == MIN:
-                    lod.accum[lv] = np.where(v < lod.accum[lv], v, lod.accum[lv])
-                else:
-                    lod.accum[lv] = np.where(v > lod.accum[lv], v, lod.accum[lv])
-            for lv in range(1, plan.nlod):
-                lod.counts[lv] += 1
-                if lod.counts[lv] >= (1 << lv):
-                    vals[lv] = emit(lod.accum[lv], lod.counts[lv])
-                    lod.counts[lv] = 0
-                    mask |= 1 << lv
+            mask = lod.fold_emit(vals, append_method)
         for lv in range(plan.nlod):
             if mask & (1 << lv):
                 lod.scatter(pool[e], lv, vals[lv])
         return mask
 
     def emit(acc, count):
-        if dt.kind == "f" and append_method == MEAN:
-            return (acc / dt.type(count)).astype(dt)
-        return acc.copy()
+        return emit_accum(acc, count, append_method)
 
     flushed = [False] * plan.K  # pool slot compressed by some flush so far
 

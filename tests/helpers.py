@@ -132,6 +132,42 @@ def decoded_chunks(shards, cps_total_of_level, codec, chunk_bytes):
     return out
 
 
+def geometry(cfg):
+    """({level: (cps_total, shard_inner_count, chunks_per_epoch)}, epoch
+    elements, chunk bytes) of a configuration, from the product's plan."""
+    s = ca.GpuStream(cfg, ca.MemSink())
+    n = s.status().nlod
+    g = {lv: s.level_geometry(lv) for lv in range(n)}
+    E = s.epoch_elements
+    chunk_bytes = s.layout.chunk_elements * cfg.np_dtype.itemsize
+    s.close()
+    return g, E, chunk_bytes
+
+
+def assert_same_shards(got, want, cps_of_level, codec, chunk_bytes, what=""):
+    """The reference's definition of equal output: the same shard keys; with
+    no codec every shard byte-identical; with lz4 / zstd every shard's index
+    fills the same slots, its CRC holds and every chunk decodes (stock
+    liblz4 / libzstd) to the same bytes. Chunks are decoded one at a time,
+    so large outputs are compared without holding decoded copies."""
+    assert set(got) == set(want), f"{what}: shard keys differ: {sorted(set(got) ^ set(want))[:8]}"
+    for k in sorted(want):
+        if codec == "none":
+            same = got[k] == want[k]  # a bool, so a failure does not diff whole shards
+            assert same, f"{what}: shard {k} differs"
+            continue
+        cps = cps_of_level(k[0])
+        gi, ri = parse_shard(got[k], cps), parse_shard(want[k], cps)
+        assert [e is None for e in gi] == [e is None for e in ri], f"{what}: index layout of shard {k} differs"
+        for slot, (ge, re_) in enumerate(zip(gi, ri)):
+            if re_ is None:
+                continue
+            a = decode(codec, got[k][ge[0]:ge[0] + ge[1]], chunk_bytes)
+            b = decode(codec, want[k][re_[0]:re_[0] + re_[1]], chunk_bytes)
+            same = a == b
+            assert same, f"{what}: decoded chunk {slot} of shard {k} differs"
+
+
 # ---- runners ------------------------------------------------------------------
 
 def run_gpu(cfg, data, pieces=1, pipeline=0, device_input=False, **options):

@@ -51,6 +51,32 @@ def test_morton_rank_brute_force(shape):
         assert O.morton_rank(list(shape), list(c)) == keys.index(zkey(c))
 
 
+@pytest.mark.parametrize("shape", [[3, 5], [4, 4], [7, 2], [5, 3, 2], [1, 9], [6, 6, 3], [13, 11],
+                                   [1000, 17], [9, 5, 7]])
+def test_morton_rank_array_brute_force(shape):
+    # the array form against the brute-force rank, every cell at once
+    cells = np.stack(np.unravel_index(np.arange(int(np.prod(shape))), shape))
+    nd = len(shape)
+    key = np.zeros(cells.shape[1], dtype=np.int64)
+    for bit in range(11):
+        for d in range(nd):
+            key |= ((cells[d] >> bit) & 1) << (bit * nd + d)
+    want = np.argsort(np.argsort(key, kind="stable"), kind="stable")
+    np.testing.assert_array_equal(O.morton_rank_array(shape, cells), want)
+
+
+@pytest.mark.parametrize("child", [[6, 5], [4, 4], [7, 3], [3, 3, 5], [25, 19], [9, 8, 5], [1, 7]])
+def test_window_ends_array_matches_scalar(child):
+    parent = [(c + 1) // 2 for c in child]
+    np.testing.assert_array_equal(O.window_ends_array(child, parent), O.window_ends(child, parent))
+    # depth-1 ranks of coordinates past the box (the z_increment carry)
+    c = np.array([[0, 1, 3, 2], [1, 1, 0, 3]])
+    for j in range(c.shape[1]):
+        inc = O.z_increment(list(c[:, j]), 2)
+        assert list(O.z_increment_array(c[:, j:j + 1], 2)[:, 0]) == inc
+        assert O.morton_rank_array([4, 4], np.array(inc)[:, None], 1)[0] == O.morton_rank([4, 4], inc, 1)
+
+
 def test_window_ends_cover_children():
     # windows of consecutive parents tile the child Morton range
     for child in ([6, 5], [4, 4], [7, 3], [3, 3, 5]):
@@ -266,3 +292,161 @@ def test_vectorized_windows_match_scalar(dtype, method):
             s = ends[i - 1] if i else 0
             want = O.reduce_window(src[b, s:e], method)
             assert got[b, i].tobytes() == np.array(want, dtype=src.dtype).tobytes(), (b, i)
+
+
+# ---- f16: exact-rational pin of the restatement's half arithmetic ---------
+#
+# The reference computes f16 LOD on the GPU only (src/gpu/lod.cu): the window
+# mean sums the halfs in a float accumulator, divides in float and converts
+# to half (:71-75, :413-528); the append-dim fold adds halfs (:847-860) and
+# the emit divides the half sum by the count (:827-845). Each step is an
+# IEEE-754 operation rounded to nearest, ties to even. The model below
+# evaluates those steps on exact rationals with its own rounding function,
+# independent of numpy's casts, and must equal the restatement bit for bit
+# on general data (all levels of the pyramid, every fold and emit).
+
+from fractions import Fraction  # noqa: E402
+
+F16 = (11, -14, 15)   # significand bits, emin, emax
+F32 = (24, -126, 127)
+INF = None
+
+
+def rne(x, fmt):
+    """x rounded to the nearest value of binary format fmt, ties to even
+    (subnormals included; None for overflow to infinity)."""
+    if x is INF:
+        return INF
+    p, emin, emax = fmt
+    if x == 0:
+        return Fraction(0)
+    sign = -1 if x < 0 else 1
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    q = Fraction(2) ** (max(e, emin) - p + 1)  # spacing of representable values near a
+    m = a / q
+    n = m.numerator // m.denominator
+    r = m - n
+    if r > Fraction(1, 2) or (r == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    v = n * q
+    if v > (2 - Fraction(2) ** (1 - p)) * Fraction(2) ** emax:
+        return INF
+    return sign * v
+
+
+def _frac(h):
+    return Fraction(float(h))
+
+
+def model_window_mean_f16(vals):
+    s = Fraction(0)
+    for v in vals:
+        s = rne(s + _frac(v), F32)
+    return rne(rne(s / len(vals), F32), F16)
+
+
+def _as_half(x):
+    return np.float16(float(x))
+
+
+@pytest.mark.parametrize("shape", [[40, 36], [17, 23], [8, 8, 6]])
+def test_f16_pyramid_every_level_exact_rational(shape):
+    # a general-data f16 epoch through every level of the restatement's
+    # pyramid; each level is the model's window means of the level below
+    ch = 4 if len(shape) == 2 else 2
+    dims = [O.ODim(4, 4, 1)] + [O.ODim(s, ch, 0, True) for s in shape]
+    plan = O.Plan(dims, 2)
+    assert plan.nlod >= 3
+    lod = O.LodState(plan, np.float16)
+    E = plan.levels[0].epoch_elements
+    x = (np.random.default_rng(sum(shape)).standard_normal(E) * 100).astype(np.float16)
+    x[::97] = np.float16(60000.0)   # large and tiny magnitudes: float sums round
+    x[5::89] = np.float16(3e-5)
+    vals = lod.reduce_levels(x, O.MEAN)
+    for lv in range(1, plan.nlod):
+        src = vals[lv - 1].reshape(plan.fixed_count, -1)
+        ends = np.asarray(lod.ends[lv - 1])
+        got = vals[lv].reshape(plan.fixed_count, -1)
+        for b in range(plan.fixed_count):
+            s = 0
+            for i, e in enumerate(ends[:got.shape[1]]):
+                want = model_window_mean_f16(src[b, s:e])
+                assert want is not INF
+                assert got[b, i] == _as_half(want) and np.signbit(got[b, i]) == (want < 0), (lv, b, i)
+                s = e
+
+
+def test_f16_fold_emit_chain_exact_rational():
+    # eight epochs of general-data level values through the restatement's
+    # fold and emit (mean, min, max) against the model's chain
+    dims = [O.ODim(16, 2, 2, True), O.ODim(20, 4, 0, True), O.ODim(12, 4, 0, True)]
+    plan = O.Plan(dims, 2)
+    assert plan.nlod >= 3
+    rng = np.random.default_rng(17)
+    for method in (O.MEAN, O.MIN, O.MAX):
+        lod = O.LodState(plan, np.float16)
+        acc = {lv: None for lv in range(1, plan.nlod)}
+        cnt = {lv: 0 for lv in range(1, plan.nlod)}
+        for epoch in range(8):
+            vals = [(rng.standard_normal(plan.fixed_count * L.lod_nelem) * 100).astype(np.float16)
+                    for L in plan.levels]
+            vals[1][::7] = np.float16(30000.0)
+            new = [v.copy() for v in vals]
+            mask = lod.fold_emit(new, method)
+            want_mask = 1
+            for lv in range(1, plan.nlod):
+                v = [_frac(h) for h in vals[lv]]
+                if cnt[lv] == 0:
+                    acc[lv] = v
+                elif method == O.MEAN:
+                    acc[lv] = [rne(a + b, F16) for a, b in zip(acc[lv], v)]
+                elif method == O.MIN:
+                    acc[lv] = [b if b < a else a for a, b in zip(acc[lv], v)]
+                else:
+                    acc[lv] = [b if b > a else a for a, b in zip(acc[lv], v)]
+                cnt[lv] += 1
+                if cnt[lv] >= (1 << lv):
+                    out = [rne(a / cnt[lv], F16) if method == O.MEAN else a for a in acc[lv]]
+                    got = new[lv]
+                    for i, w in enumerate(out):
+                        assert w is not INF
+                        assert got[i] == _as_half(w), (method, epoch, lv, i)
+                    cnt[lv] = 0
+                    want_mask |= 1 << lv
+            assert mask == want_mask
+            for lv in range(1, plan.nlod):
+                if acc[lv] is not None and all(a is not INF for a in acc[lv]):
+                    np.testing.assert_array_equal(lod.accum[lv], np.array([float(a) for a in acc[lv]],
+                                                                          dtype=np.float16))
+
+
+def test_f16_model_catches_a_half_accumulator():
+    # the model distinguishes the reference's float accumulator from a
+    # half-precision one on these inputs (a planted mistake on the CPU)
+    vals = np.array([2048, 1, 1, 1], dtype=np.float16)
+    good = model_window_mean_f16(vals)
+    s = Fraction(0)
+    for v in vals:
+        s = rne(s + _frac(v), F16)
+    bad = rne(s / len(vals), F16)
+    assert good != bad
+    got = O.reduce_windows(vals.reshape(1, -1), np.array([4]), O.MEAN)[0, 0]
+    assert got == _as_half(good)
+
+
+def test_rne_matches_ieee_on_known_values():
+    # closed forms of the rounding itself: half max, the smallest subnormal,
+    # ties to even at 2049 -> 2048 and 2051 -> 2052, overflow
+    assert rne(Fraction(65504), F16) == 65504
+    assert rne(Fraction(65520), F16) is INF
+    assert rne(Fraction(2049), F16) == 2048
+    assert rne(Fraction(2051), F16) == 2052
+    assert rne(Fraction(1, 2 ** 24), F16) == Fraction(1, 2 ** 24)
+    assert rne(Fraction(1, 2 ** 25), F16) == 0  # tie between 0 and 2^-24: even (0)
+    assert rne(Fraction(3, 2 ** 26), F16) == Fraction(1, 2 ** 24)
+    assert rne(Fraction(1, 3), F32) == Fraction(11184811, 2 ** 25)
