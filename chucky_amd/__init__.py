@@ -150,6 +150,8 @@ _sig("tile_stream_gpu_status", TileStreamStatus, _P)
 _sig("tile_stream_gpu_get_metrics", StreamMetrics, _P)
 _sig("tile_stream_gpu_memory_estimate", C.c_int, C.POINTER(TileStreamConfiguration),
      C.POINTER(MemoryInfo))
+_sig("tile_stream_gpu_memory_estimate_ex", C.c_int, C.POINTER(TileStreamConfiguration),
+     C.POINTER(StreamOptions), C.POINTER(MemoryInfo))
 _sig("tile_stream_gpu_advise_chunk_sizes", C.c_int, C.POINTER(TileStreamConfiguration),
      C.c_size_t, C.POINTER(C.c_uint8), C.c_size_t)
 _sig("tile_stream_gpu_append_device", WriterResult, _P, _P, C.c_size_t)
@@ -195,6 +197,7 @@ _sig("chucky_device_free", None, _P)
 _sig("chucky_memcpy_to_device", C.c_int, _P, _P, C.c_size_t)
 _sig("chucky_memcpy_to_host", C.c_int, _P, _P, C.c_size_t)
 _sig("chucky_device_synchronize", C.c_int)
+_sig("chucky_measure_link_gbps", C.c_double, C.c_size_t, C.c_int, C.c_int)
 _sig("chucky_synth_sensor_frames", C.c_int, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
      C.c_double, C.POINTER(C.c_float), C.c_uint32)
 _sig("chucky_synth_fill", C.c_int, _P, C.c_uint64, C.c_int, C.c_uint64, C.c_uint64)

@@ -161,10 +161,19 @@ class DiscardSink:
         self.close()
 
 
-def memory_estimate(cfg: Config) -> MemoryInfo:
+def memory_estimate(cfg: Config, pipeline: Optional[int] = None) -> MemoryInfo:
+    """tile_stream_gpu_memory_estimate; with a pipeline, the estimate of what
+    that pipeline allocates (tile_stream_gpu_memory_estimate_ex)."""
     info = MemoryInfo()
     c = cfg.to_c()
-    if lib.tile_stream_gpu_memory_estimate(C.byref(c), C.byref(info)):
+    if pipeline is None:
+        rc = lib.tile_stream_gpu_memory_estimate(C.byref(c), C.byref(info))
+    else:
+        opts = StreamOptions()
+        lib.tile_stream_gpu_options_default(C.byref(opts))
+        opts.pipeline = pipeline
+        rc = lib.tile_stream_gpu_memory_estimate_ex(C.byref(c), C.byref(opts), C.byref(info))
+    if rc:
         raise ValueError("invalid configuration")
     return info
 

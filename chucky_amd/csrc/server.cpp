@@ -181,8 +181,13 @@ int chucky_server_submit(struct chucky_server* srv, const struct tile_stream_con
                          struct shard_sink* sink, const void* data, size_t nbytes,
                          int data_is_device, uint64_t* request_id)
 {
+  // admission counts what the request's pipeline allocates
   tile_stream_memory_info info;
-  if (!srv || !config || !sink || tile_stream_gpu_memory_estimate(config, &info)) {
+  tile_stream_gpu_options o;
+  tile_stream_gpu_options_default(&o);
+  if (srv)
+    o.pipeline = srv->pipeline;
+  if (!srv || !config || !sink || tile_stream_gpu_memory_estimate_ex(config, &o, &info)) {
     chucky::set_error("chucky_server_submit: invalid request");
     return 1;
   }

@@ -201,6 +201,12 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
     set_error("aggregate: output exceeds the aggregate buffer");
     return 1;
   }
+  // Page-aligned streams write whole pages: the bytes between a run's end and
+  // its page boundary are written too. The reference leaves them as whatever
+  // its malloc'd aggregate buffer held (src/cpu/stream.c:105); here they are
+  // zeros.
+  if (s->plan.config.shard_alignment)
+    HIPCHECK(hipMemsetAsync(s->d_agg, 0, copy, s->st_comp));
   HIPCHECK(launch_aggregate_gather(src, s->d_gather, s->d_perm, n_items, s->d_offsets, s->d_agg,
                                    s->st_comp));
   HIPCHECK(hipEventRecord(e1, s->st_comp));
@@ -543,9 +549,16 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
   const uint64_t maxc = s->plan.max_cursor_elements;
   while (src < end) {
     if (maxc && s->cursor >= maxc) {
-      const int rc = flush_all(s);
-      s->flushed = 1;
-      return rc ? writer_error_at(src, end) : writer_finished_at(src, end);
+      // capacity reached: everything is delivered once and the append says
+      // finished; the caller's own flush still succeeds afterwards, as in
+      // the reference (src/cpu/stream.c:545-548, 764-771)
+      if (!s->at_capacity) {
+        const int rc = flush_all(s);
+        s->at_capacity = 1;
+        if (rc)
+          return writer_error_at(src, end);
+      }
+      return writer_finished_at(src, end);
     }
     const uint64_t in_left = (uint64_t)(end - src) / bpe;
     if (in_left == 0)
@@ -583,8 +596,14 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
         if (s->stage_fill == 0 && hipEventSynchronize(s->ev_stage_free[i]) != hipSuccess)
           return writer_error_at(src, end);
         const size_t payload = (size_t)std::min<uint64_t>(s->stage_cap - s->stage_fill, bytes - written);
+        if (!s->copy_pool)
+          s->copy_pool.reset(new (std::nothrow) CopyPool(default_copy_threads()));
+        if (!s->copy_pool) {
+          set_error("append: staging copy threads could not start");
+          return writer_error_at(src, end);
+        }
         const auto t0 = std::chrono::steady_clock::now();
-        std::memcpy(s->h_stage[i] + s->stage_fill, src + written, payload);
+        s->copy_pool->copy(s->h_stage[i] + s->stage_fill, src + written, payload);
         add_metric(s->metrics.memcpy,
                    std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count(),
                    (double)payload, (double)payload);
@@ -616,7 +635,7 @@ writer_result gpu_flush(writer* self)
     return writer_finished_at(nullptr, nullptr);
   if (hipSetDevice(s->device) != hipSuccess)
     return writer_error();
-  const int rc = flush_all(s);
+  const int rc = s->at_capacity ? 0 : flush_all(s);
   s->flushed = 1;
   return rc ? writer_error() : writer_ok();
 }
@@ -959,6 +978,7 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   s->cursor = 0;
   s->batch_acc = 0;
   s->flushed = 0;
+  s->at_capacity = 0;
   std::fill(s->slot_compressed.begin(), s->slot_compressed.end(), (uint8_t)0);
   s->stage_fill = 0;
   std::fill(std::begin(s->append_counts), std::end(s->append_counts), 0u);
@@ -1059,6 +1079,41 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
   info->max_output_size = p.max_output_size;
   info->nlod = p.nlod;
   info->epochs_per_batch = K;
+  return 0;
+}
+
+int tile_stream_gpu_memory_estimate_ex(const struct tile_stream_configuration* config,
+                                       const struct tile_stream_gpu_options* opts,
+                                       struct tile_stream_memory_info* info)
+{
+  if (tile_stream_gpu_memory_estimate(config, info))
+    return 1;
+  // the batched pipeline (which aligned streams do not use) adds, per host
+  // slot, one epoch's worst-case bytes of every level in pinned memory and
+  // in device staging, and per pool slot its aggregate LUTs and sizes
+  if (!opts || opts->pipeline != CHUCKY_PIPELINE_BATCHED || config->shard_alignment)
+    return 0;
+  Plan p;
+  std::string err;
+  if (build_plan(config, 1, gpu_max_output, &p, &err))
+    return 1;
+  const uint32_t K = p.epochs_per_batch;
+  const uint64_t R = std::min<uint32_t>(K, 4);
+  const uint64_t cb = p.levels[0].layout.chunk_elements * p.bpe;
+  const uint64_t per = config->codec.id == CODEC_NONE ? cb : p.max_output_size;
+  uint64_t epoch_bytes = 0, luts = 0, sizes = 0;
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    const LevelInfo& L = p.levels[lv];
+    epoch_bytes += align_up(L.agg.chunks_per_epoch * per, 256);
+    luts += L.agg.chunks_per_epoch * 8 + (2 * L.agg.covering_count + 1) * 8;
+    sizes += (2 * L.agg.covering_count + 1) * 8;
+  }
+  const bool dma = opts->batched_d2h != 1;
+  const uint64_t device_extra = (dma ? R * epoch_bytes : 0) + (uint64_t)K * luts;
+  const uint64_t host_extra = R * (epoch_bytes + sizes);
+  info->aggregate_bytes += device_extra;
+  info->device_bytes += device_extra;
+  info->host_pinned_bytes += host_extra;
   return 0;
 }
 

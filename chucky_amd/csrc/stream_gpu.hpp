@@ -2,6 +2,7 @@
 #pragma once
 
 #include "chucky_amd.h"
+#include "host_copy.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 #include "shards.hpp"
@@ -11,6 +12,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <mutex>
@@ -40,6 +42,7 @@ struct tile_stream_gpu
   hipEvent_t ev_linear_free = nullptr; // LOD gather of the last epoch has read d_linear
   size_t stage_fill = 0;
   int stage_cur = 0;
+  std::unique_ptr<chucky::CopyPool> copy_pool; // staging copy threads, started at the first host append
 
   // chunk pool: K epochs x total_chunks x chunk_stride elements
   uint8_t* d_pool = nullptr;
@@ -96,6 +99,7 @@ struct tile_stream_gpu
   uint32_t batch_acc = 0;
   uint32_t masks[MAX_BATCH_EPOCHS] = {};
   int flushed = 0;
+  int at_capacity = 0; // a bounded stream reached its capacity and delivered everything
   std::vector<chucky::ShardState> shards;
   std::vector<io_event> io_done;
   stream_metrics metrics{};

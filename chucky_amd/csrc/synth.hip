@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <string>
 #include <vector>
 
 namespace chucky {
 void set_error(const std::string& msg);
+hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
 }
 
 namespace {
@@ -140,6 +142,48 @@ int chucky_memcpy_to_host(void* h, const void* d, size_t n)
 }
 
 int chucky_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
+
+double chucky_measure_link_gbps(size_t bytes, int reps, int to_device)
+{
+  void* h = nullptr;
+  void* d = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  double gbps = -1;
+  float ms = 0;
+  int dev = 0;
+  if (reps < 1 || hipGetDevice(&dev) != hipSuccess || chucky::host_alloc_near_gpu(&h, bytes, dev) != hipSuccess || hipMalloc(&d, bytes) != hipSuccess ||
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
+      hipEventCreate(&b) != hipSuccess)
+    goto done;
+  std::memset(h, 1, bytes);
+  // one untimed copy, then `reps` timed back to back
+  for (int r = 0; r <= reps; ++r) {
+    if (r == 1 && hipEventRecord(a, st) != hipSuccess)
+      goto done;
+    if (hipMemcpyAsync(to_device ? d : h, to_device ? h : d, bytes,
+                       to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, st) != hipSuccess)
+      goto done;
+  }
+  if (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+      hipEventElapsedTime(&ms, a, b) != hipSuccess || ms <= 0)
+    goto done;
+  gbps = (double)bytes * reps / (ms * 1e-3) / 1e9;
+done:
+  if (a)
+    (void)hipEventDestroy(a);
+  if (b)
+    (void)hipEventDestroy(b);
+  if (st)
+    (void)hipStreamDestroy(st);
+  if (d)
+    (void)hipFree(d);
+  if (h)
+    (void)hipHostFree(h);
+  if (gbps < 0)
+    chucky::set_error("chucky_measure_link_gbps failed");
+  return gbps;
+}
 
 int chucky_synth_sensor_frames(void* d_dst, uint64_t frames, uint32_t height, uint32_t width,
                                uint64_t seed, double lambda, const float* blobs, uint32_t nblob)

@@ -458,7 +458,7 @@ int pwrite_all(int fd, const void* buf, size_t n, uint64_t off)
     if (w < 0) {
       if (errno == EINTR)
         continue;
-      return 1;
+      return errno ? errno : 1;
     }
     p += w;
     n -= (size_t)w;
@@ -486,6 +486,8 @@ struct zarr_fs_sink
   size_t page = 4096;
   std::atomic<uint64_t> queued{ 0 }, retired{ 0 };
   std::atomic<int> io_error{ 0 };
+  std::mutex err_mu;
+  std::string err_msg; // the first failed write, for the caller's error message
   int rank = 0;
   std::vector<DimExtent> ext;
   uint64_t shard_inner_count = 1;
@@ -500,6 +502,26 @@ struct zarr_fs_sink
 namespace {
 
 zarr_fs_sink* fs_of(shard_sink* s) { return reinterpret_cast<zarr_fs_sink*>(s); }
+
+// Runs on the I/O worker: records the first failed write.
+void io_failed(zarr_fs_sink* zs, int err, const void* buf, size_t n, uint64_t off)
+{
+  std::lock_guard<std::mutex> lk(zs->err_mu);
+  if (zs->err_msg.empty())
+    zs->err_msg = "zarr_fs_sink: pwrite of " + std::to_string(n) + " bytes at offset " +
+                  std::to_string(off) + " (buffer address mod 4096 = " +
+                  std::to_string((uintptr_t)buf % 4096) + ") failed: " + std::strerror(err);
+  zs->io_error = 1;
+}
+
+int io_status(zarr_fs_sink* zs)
+{
+  if (!zs->io_error.load())
+    return 0;
+  std::lock_guard<std::mutex> lk(zs->err_mu);
+  set_error(zs->err_msg.empty() ? std::string("zarr_fs_sink: write failed") : zs->err_msg);
+  return 1;
+}
 
 int fs_write(shard_writer* self, uint64_t offset, const void* beg, const void* end)
 {
@@ -516,12 +538,12 @@ int fs_write(shard_writer* self, uint64_t offset, const void* beg, const void* e
   const int fd = w->fd;
   zs->queued += n;
   zs->queue->post([zs, fd, copy, n, offset] {
-    if (pwrite_all(fd, copy, n, offset))
-      zs->io_error = 1;
+    if (const int e = pwrite_all(fd, copy, n, offset))
+      io_failed(zs, e, copy, n, offset);
     std::free(copy);
     zs->retired += n;
   });
-  return zs->io_error.load() ? 1 : 0;
+  return io_status(zs);
 }
 
 int fs_write_direct(shard_writer* self, uint64_t offset, const void* beg, const void* end)
@@ -534,11 +556,11 @@ int fs_write_direct(shard_writer* self, uint64_t offset, const void* beg, const 
   const int fd = w->fd;
   zs->queued += n;
   zs->queue->post([zs, fd, beg, n, offset] {
-    if (pwrite_all(fd, beg, n, offset))
-      zs->io_error = 1;
+    if (const int e = pwrite_all(fd, beg, n, offset))
+      io_failed(zs, e, beg, n, offset);
     zs->retired += n;
   });
-  return zs->io_error.load() ? 1 : 0;
+  return io_status(zs);
 }
 
 int fs_finalize(shard_writer* self)

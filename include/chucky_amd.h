@@ -43,6 +43,14 @@ struct tile_stream_gpu_options
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);
 
+/* tile_stream_gpu_memory_estimate for the pipeline opts selects: the batched
+ * pipeline's pinned host slots, their device staging and its per-slot
+ * aggregate tables are included (device_bytes, aggregate_bytes,
+ * host_pinned_bytes). NULL opts = the plain pipeline. 0 on success. */
+int tile_stream_gpu_memory_estimate_ex(const struct tile_stream_configuration* config,
+                                       const struct tile_stream_gpu_options* opts,
+                                       struct tile_stream_memory_info* info);
+
 struct tile_stream_gpu* tile_stream_gpu_create_ex(const struct tile_stream_configuration* config,
                                                   struct shard_sink* sink,
                                                   const struct tile_stream_gpu_options* opts);
@@ -172,6 +180,10 @@ void chucky_device_free(void* p);
 int chucky_memcpy_to_device(void* d_dst, const void* h_src, size_t n);
 int chucky_memcpy_to_host(void* h_dst, const void* d_src, size_t n);
 int chucky_device_synchronize(void);
+/* DMA rate of `bytes` between pinned host memory (on the GPU's NUMA node)
+ * and device memory of the current device: GB/s over `reps` back-to-back
+ * copies after one untimed copy; to_device 1 = host to device. -1 on error. */
+double chucky_measure_link_gbps(size_t bytes, int reps, int to_device);
 
 /* Synthetic 12-bit sensor frames (u16, frames x height x width, row-major) in
  * device memory: per pixel a Poisson(lambda) background drawn by inversion of

@@ -47,6 +47,13 @@ def lib():
         _lib.writer_append.argtypes = [C.POINTER(Writer), Slice]
         _lib.writer_flush.restype = WriterResult
         _lib.writer_flush.argtypes = [C.POINTER(Writer)]
+        from chucky_amd import MultiarrayWriter
+        _lib.multiarray_tile_stream_cpu_create.restype = P
+        _lib.multiarray_tile_stream_cpu_create.argtypes = [C.c_int, C.POINTER(TileStreamConfiguration),
+                                                           C.POINTER(C.c_void_p), C.c_int]
+        _lib.multiarray_tile_stream_cpu_destroy.argtypes = [P]
+        _lib.multiarray_tile_stream_cpu_writer.restype = C.POINTER(MultiarrayWriter)
+        _lib.multiarray_tile_stream_cpu_writer.argtypes = [P]
     return _lib
 
 
@@ -82,6 +89,48 @@ class RefCpuStream:
     def close(self):
         if self.h:
             lib().tile_stream_cpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class RefMultiArray:
+    """The reference's multiarray_tile_stream_cpu_* writer
+    (src/multiarray.cpu.h:31-46), same calls as chucky_amd.GpuMultiArray."""
+
+    def __init__(self, cfgs, sinks):
+        from chucky_amd import TileStreamConfiguration
+        L = lib()
+        self._c = [c.to_c() for c in cfgs]
+        arr = (TileStreamConfiguration * len(cfgs))()
+        for i, c in enumerate(self._c):
+            arr[i] = c
+        self._arr = arr
+        ss = (C.c_void_p * len(sinks))(*[s.shard_sink for s in sinks])
+        self._ss = ss
+        self._sinks = list(sinks)  # the sinks must outlive the writer
+        self.h = L.multiarray_tile_stream_cpu_create(len(cfgs), arr, ss, 0)
+        if not self.h:
+            raise RuntimeError("multiarray_tile_stream_cpu_create failed")
+        self.w = L.multiarray_tile_stream_cpu_writer(self.h)
+
+    def update(self, index: int, data):
+        """Returns (error code, bytes consumed)."""
+        import numpy as np
+        from chucky_amd import Slice
+        buf = np.ascontiguousarray(data)
+        beg = buf.ctypes.data
+        r = self.w.contents.update(self.w, index, Slice(beg, beg + buf.nbytes))
+        rest = (r.rest.end or 0) - (r.rest.beg or 0) if r.rest.beg else 0
+        return r.error, buf.nbytes - rest
+
+    def flush(self) -> int:
+        return self.w.contents.flush(self.w).error
+
+    def close(self):
+        if self.h:
+            lib().multiarray_tile_stream_cpu_destroy(self.h)
             self.h = None
 
     def __del__(self):

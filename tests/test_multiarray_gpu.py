@@ -83,3 +83,61 @@ def test_multiarray_switch_mid_epoch_not_flushable():
     for i, (c, d) in enumerate(zip(cfgs, datas)):
         want, _, _ = run_gpu(c, d, pieces=3)
         assert sinks[i].shards() == want, i
+
+
+def _script(datas, epochs, seed):
+    """A seeded sequence of (array, start, stop): whole epochs, partial
+    epochs, switches away from a partial epoch, and appends past a bounded
+    array's capacity."""
+    rng = np.random.default_rng(seed)
+    pos = [0] * len(datas)
+    out = []
+    for _ in range(40):
+        i = int(rng.integers(len(datas)))
+        e = epochs[i]
+        kind = rng.integers(4)
+        n = e if kind < 2 else (e // 2 if kind == 2 else 3 * e)
+        out.append((i, pos[i], pos[i] + n))
+        pos[i] += n
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_multiarray_matches_reference_multiarray(seed):
+    # the GPU multi-array writer and the reference's multiarray_tile_stream_cpu
+    # (oracle/_ref) take the same scripted updates: the same error codes
+    # (including not_flushable for a switch away from a partial epoch and
+    # finished past a bounded array's capacity), the same bytes consumed,
+    # and after the final flush (active array mid-epoch) the same shards
+    from oracle import ref
+    from tests.helpers import assert_same_shards, geometry
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    cfgs = [c for c, _ in ARRAYS]
+    datas = [np.ascontiguousarray(make()).reshape(-1) for _, make in ARRAYS]
+    datas = [np.concatenate([d, d, d]) for d in datas]  # more than capacity
+    epochs = []
+    for c in cfgs:
+        s = ca.GpuStream(c, ca.MemSink())
+        epochs.append(s.epoch_elements)
+        s.close()
+    gsinks = [ca.MemSink() for _ in cfgs]
+    rsinks = [ca.MemSink() for _ in cfgs]
+    g = ca.GpuMultiArray(cfgs, gsinks)
+    r = ref.RefMultiArray(cfgs, rsinks)
+    codes = set()
+    for i, a, b in _script(datas, epochs, seed):
+        part = datas[i][a:b]
+        ge, gu = g.update(i, part)
+        re_, ru = r.update(i, part)
+        assert (ge, gu) == (re_, ru), (i, a, b, ge, gu, re_, ru, ca.last_error())
+        codes.add(ge)
+    assert g.flush() == r.flush()
+    g.close()
+    r.close()
+    assert {0, 3} <= codes, codes
+    for i, c in enumerate(cfgs):
+        geo, _, cb = geometry(c)
+        assert_same_shards(gsinks[i].shards(), rsinks[i].shards(), lambda lv: geo[lv][0], c.codec, cb,
+                           f"array {i}")

@@ -68,7 +68,9 @@ def test_budget_bounds_concurrency():
     # a budget below any two requests: they run one at a time, and a request
     # larger than the whole budget still runs (alone)
     reqs = _requests()[:3]
-    est = [ca.memory_estimate(c).device_bytes for c, _ in reqs]
+    est = [ca.memory_estimate(c, ca.PIPELINE_BATCHED).device_bytes for c, _ in reqs]
+    plain = [ca.memory_estimate(c).device_bytes for c, _ in reqs]
+    assert all(e > p for e, p in zip(est, plain))  # the batched pipeline's staging is counted
     srv = Server(hbm_budget_bytes=min(est), max_concurrent=8)
     sinks = []
     for c, d in reqs:
@@ -82,3 +84,25 @@ def test_budget_bounds_concurrency():
     lat = srv.latencies()
     assert len(lat) == sum(r.epochs for r in reps) and min(lat) > 0
     srv.close()
+
+
+def test_request_past_capacity_finishes_cleanly():
+    # a bounded request handed more input than its capacity: the stream
+    # delivers everything at capacity and says finished; the request counts
+    # as a success and its shards equal the reference's on the same input
+    from oracle import ref
+    from tests.helpers import assert_same_shards, geometry, run_ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    cfg = ca.Config([Dim(24, 8, 2), Dim(32, 16, 1), Dim(32, 16, 2)], dtype="u16", codec="zstd")
+    v = xor_volume((40, 32, 32))  # 16 frames past the 24-frame capacity
+    srv = Server(hbm_budget_bytes=8 << 30, max_concurrent=2, pipeline=ca.PIPELINE_BATCHED)
+    sink = ca.MemSink()
+    srv.submit(cfg, sink, v)
+    assert srv.wait() == 0, ca.last_error()
+    reps = srv.reports()
+    srv.close()
+    assert len(reps) == 1 and reps[0].status == 0
+    g, _, cb = geometry(cfg)
+    rs, _, _ = run_ref(cfg, v[:24])
+    assert_same_shards(sink.shards(), rs, lambda lv: g[lv][0], "zstd", cb, "past capacity")
