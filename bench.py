@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--frames", type=int, default=512, help="frames of the sensor / lod workloads")
     ap.add_argument("--workload", choices=["multiscale", "multiscale6", "sensor", "volume", "lod", "serving"],
                     default="multiscale")
+    ap.add_argument("--pipeline", choices=["plain", "batched", "both"], default="both",
+                    help="time one pipeline alone (profiling runs) or both")
     ap.add_argument("--device-input", action="store_true",
                     help="time input already resident in HBM instead of host-memory appends")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads beside the headline")
@@ -430,19 +432,21 @@ def main():
     data, host = make_input(args.workload, shape, True)
     h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)
     d2h_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 0)
-    runs = run(cfg, desc, data, host, args.steps, args.warmup, ["plain", "batched"], host_input, options)
-    value, head = summarize(runs["batched"], args.steps)
-    plain = summarize(runs["plain"], args.steps)[1]
-    roofline = compress_roofline(runs["batched"], args.workload)
-    if roofline is None and lod_rates(runs["batched"]).get("lod_gather"):
-        g = lod_rates(runs["batched"])["lod_gather"]
+    pipes = ["plain", "batched"] if args.pipeline == "both" else [args.pipeline]
+    runs = run(cfg, desc, data, host, args.steps, args.warmup, pipes, host_input, options)
+    lead = "batched" if "batched" in runs else "plain"
+    value, head = summarize(runs[lead], args.steps)
+    roofline = compress_roofline(runs[lead], args.workload)
+    if roofline is None and lod_rates(runs[lead]).get("lod_gather"):
+        g = lod_rates(runs[lead])["lod_gather"]
         roofline = {"bound": "hbm", "kernel": "tiles_k (LOD gather)", "achieved": g["achieved_GBps"],
                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": g["frac_of_hbm_peak"],
                     "traffic": measured_traffic(args.workload, "tiles_k"),
                     "avg_launch_ms": g["avg_launch_ms"], "bytes_per_launch": g["bytes_per_launch"],
                     "note": "epoch bytes read + Morton bytes written per launch / event-timed launch duration"}
 
-    exact = bit_exact(cfg, data, host) if args.workload in ("multiscale", "multiscale6", "volume") else None
+    exact = (bit_exact(cfg, data, host) if args.workload in ("multiscale", "multiscale6", "volume")
+             and args.pipeline == "both" else None)
     cpu = None
     if not args.no_cpu_baseline:
         if args.workload == "sensor":
@@ -468,28 +472,29 @@ def main():
         "dtype": "u16",
         "data": "synthetic, " + ("appended from host memory (pinned staging + H2D inside the timed region)"
                                  if host_input else "resident in HBM before the timed region"),
-        "config": {"workload": desc, "pipeline": "batched", "input_GB": round(data.nbytes / 1e9, 4),
+        "config": {"workload": desc, "pipeline": lead, "input_GB": round(data.nbytes / 1e9, 4),
                    "input_from": "host memory" if host_input else "HBM"},
         "GiB_per_s": head["GiBps"],
         "compression_ratio": head["ratio"],
         "p50_epoch_latency_ms": head["p50_epoch_latency_ms"],
         "p99_epoch_latency_ms": head["p99_epoch_latency_ms"],
-        "batched_over_plain": round(runs["plain"]["seconds"] / runs["batched"]["seconds"], 3),
-        "pipelines": {"plain": plain, "batched": head},
+        "batched_over_plain": (round(runs["plain"]["seconds"] / runs["batched"]["seconds"], 3)
+                               if len(runs) == 2 else None),
+        "pipelines": {p: summarize(runs[p], args.steps)[1] for p in runs},
         "bit_exact": exact,
         "link": {"h2d_peak_GBps": round(h2d_peak, 2), "d2h_peak_GBps": round(d2h_peak, 2),
                  "value_frac_of_h2d_peak": round(value / h2d_peak, 4) if h2d_peak > 0 else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    t = transpose_rate(runs["batched"])
+    t = transpose_rate(runs[lead])
     if t:
         line["transpose"] = t
-    lod = lod_rates(runs["batched"])
+    lod = lod_rates(runs[lead])
     if lod:
         line["lod"] = lod
     data.free()
-    if not args.no_extras and args.workload == "multiscale":
+    if not args.no_extras and args.workload == "multiscale" and args.pipeline == "both":
         extras = {}
         extras["multiscale_hbm_input"] = leg("multiscale", args.steps, args.warmup, False)
         extras["multiscale6_host_input"] = leg("multiscale6", max(3, args.steps // 4), 1, True)

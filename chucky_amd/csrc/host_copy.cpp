@@ -14,6 +14,7 @@ CopyPool::CopyPool(unsigned threads)
 
 CopyPool::~CopyPool()
 {
+  wait();
   {
     std::lock_guard<std::mutex> lk(mu_);
     stop_ = true;
@@ -50,8 +51,9 @@ void CopyPool::run()
   }
 }
 
-void CopyPool::copy(void* dst, const void* src, size_t n)
+void CopyPool::start(void* dst, const void* src, size_t n)
 {
+  wait();
   if (workers_.empty() || n < 2 * kPiece) {
     std::memcpy(dst, src, n);
     return;
@@ -65,10 +67,24 @@ void CopyPool::copy(void* dst, const void* src, size_t n)
     busy_ = (unsigned)workers_.size();
     ++generation_;
   }
+  pending_ = true;
   start_.notify_all();
+}
+
+void CopyPool::wait()
+{
+  if (!pending_)
+    return;
   work();
   std::unique_lock<std::mutex> lk(mu_);
   done_.wait(lk, [&] { return busy_ == 0; });
+  pending_ = false;
+}
+
+void CopyPool::copy(void* dst, const void* src, size_t n)
+{
+  start(dst, src, n);
+  wait();
 }
 
 unsigned default_copy_threads()

@@ -24,6 +24,11 @@ public:
 
   // dst[0, n) = src[0, n); returns once every piece is copied.
   void copy(void* dst, const void* src, size_t n);
+  // The same copy, begun on the helper threads; the caller may do other
+  // work and must call wait() before touching dst or src again.
+  void start(void* dst, const void* src, size_t n);
+  // Joins the copy begun by start() and returns once it is complete.
+  void wait();
   unsigned threads() const { return (unsigned)workers_.size(); }
 
 private:
@@ -42,6 +47,7 @@ private:
   const uint8_t* src_ = nullptr;
   size_t n_ = 0;
   std::atomic<size_t> next_{ 0 };
+  bool pending_ = false; // a copy begun by start() not yet joined
 };
 
 // Helper threads for staging copies: the CHUCKY_COPY_THREADS environment

@@ -964,7 +964,7 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
   // each pass is timed by events that its first and last kernel record
   // (start and end of execution) and credited its algorithmic bytes
   hipEvent_t ta = nullptr, tb = nullptr;
-  if (timing_pair(&ta, &tb) ||
+  if (timing_pair(s, &ta, &tb) ||
       move_launch<T, true>(s->luts.gather_tiles, s->d_gather_tiles, s->d_gather_edge, lod, linear,
                            s->d_gather_fixed, n0, p.fixed_count, st, ta, tb) != hipSuccess)
     return 1;
@@ -975,7 +975,7 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
       last_reduce = l;
   if (last_reduce >= 0) {
     double rin = 0, rout = 0;
-    if (timing_pair(&ta, &tb) ||
+    if (timing_pair(s, &ta, &tb) ||
         pyramid_launch<T>(p, p.config.reduce_method, lod, s->d_ends, 0, last_reduce + 1, st, ta, tb, &rin,
                           &rout) != hipSuccess)
       return 1;
@@ -1031,7 +1031,7 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
         lvs.push_back(lv);
         sbytes += nb * p.levels[lv].lod_nelem * bpe;
       }
-    if (timing_pair(&ta, &tb) || scatter_levels<T>(s, lvs, lod, pool, st, ta, tb) != hipSuccess)
+    if (timing_pair(s, &ta, &tb) || scatter_levels<T>(s, lvs, lod, pool, st, ta, tb) != hipSuccess)
       return 1;
     timing_add(s, ta, tb, &stream_metrics::lod_morton_chunk, sbytes, sbytes);
   }

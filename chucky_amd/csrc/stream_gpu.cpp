@@ -90,6 +90,27 @@ float event_ms(hipEvent_t a, hipEvent_t b)
   return ms;
 }
 
+// Timing events are recycled: creating one costs the host tens of
+// microseconds, several times per epoch.
+hipEvent_t take_timing(tile_stream_gpu* s)
+{
+  if (!s->timing_free.empty()) {
+    hipEvent_t e = s->timing_free.back();
+    s->timing_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess)
+    return nullptr;
+  return e;
+}
+
+void give_timing(tile_stream_gpu* s, hipEvent_t e)
+{
+  if (e)
+    s->timing_free.push_back(e);
+}
+
 LiftedSpace space_of(const tile_stream_layout& L)
 {
   LiftedSpace sp{};
@@ -135,9 +156,11 @@ int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
   const auto codec = s->plan.config.codec;
   if (codec.id == CODEC_NONE || n == 0)
     return 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHECK(hipEventCreate(&e0));
-  HIPCHECK(hipEventCreate(&e1));
+  hipEvent_t e0 = take_timing(s), e1 = take_timing(s);
+  if (!e0 || !e1) {
+    set_error("event creation failed");
+    return 1;
+  }
   HIPCHECK(hipEventRecord(e0, s->st_comp));
   if (codec.id == CODEC_LZ4_NON_STANDARD) {
     HIPCHECK(launch_lz4(s->d_pool, s->chunk_stride_bytes, s->chunk_bytes, n, s->d_comp,
@@ -151,8 +174,8 @@ int compress_batch(tile_stream_gpu* s, uint32_t n_epochs)
   HIPCHECK(hipEventRecord(e1, s->st_comp));
   HIPCHECK(hipEventSynchronize(e1));
   add_metric(s->metrics.compress, event_ms(e0, e1), (double)n * s->chunk_bytes, 0);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  give_timing(s, e0);
+  give_timing(s, e1);
   return 0;
 }
 
@@ -174,10 +197,11 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
   HIPCHECK(hipMemcpyAsync(s->d_gather, hg, n_items * 4, hipMemcpyHostToDevice, s->st_comp));
   HIPCHECK(hipMemcpyAsync(s->d_perm, hp, n_items * 4, hipMemcpyHostToDevice, s->st_comp));
 
-  hipEvent_t e0, e1, e2;
-  HIPCHECK(hipEventCreate(&e0));
-  HIPCHECK(hipEventCreate(&e1));
-  HIPCHECK(hipEventCreate(&e2));
+  hipEvent_t e0 = take_timing(s), e1 = take_timing(s), e2 = take_timing(s);
+  if (!e0 || !e1 || !e2) {
+    set_error("event creation failed");
+    return 1;
+  }
   HIPCHECK(hipEventRecord(e0, s->st_comp));
   SegmentSource src = segment_source(s);
   if (empty) {
@@ -221,9 +245,9 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
   HIPCHECK(hipEventSynchronize(e2));
   add_metric(s->metrics.aggregate, event_ms(e0, e1), (double)total, (double)total);
   add_metric(s->metrics.d2h, event_ms(e1, e2), (double)copy, (double)copy);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
+  give_timing(s, e0);
+  give_timing(s, e1);
+  give_timing(s, e2);
 
   const auto t0 = std::chrono::steady_clock::now();
   AggView v{ s->h_agg, h_off, h_cs };
@@ -243,7 +267,11 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
 // once the queue has passed them.
 int time_begin(tile_stream_gpu* s, hipEvent_t* a)
 {
-  HIPCHECK(hipEventCreate(a));
+  *a = take_timing(s);
+  if (!*a) {
+    set_error("event creation failed");
+    return 1;
+  }
   HIPCHECK(hipEventRecord(*a, s->st_comp));
   return 0;
 }
@@ -251,8 +279,11 @@ int time_begin(tile_stream_gpu* s, hipEvent_t* a)
 int time_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,
              double out_bytes)
 {
-  hipEvent_t b;
-  HIPCHECK(hipEventCreate(&b));
+  hipEvent_t b = take_timing(s);
+  if (!b) {
+    set_error("event creation failed");
+    return 1;
+  }
   HIPCHECK(hipEventRecord(b, s->st_comp));
   s->pending_timing.push_back({ a, b, in_bytes, out_bytes, which });
   return 0;
@@ -273,8 +304,8 @@ void collect_scatter_ready(tile_stream_gpu* s)
     auto& t = s->pending_timing[i];
     if (hipEventQuery(t.b) == hipSuccess) {
       add_metric(s->metrics.*t.which, event_ms(t.a, t.b), t.in_bytes, t.out_bytes);
-      (void)hipEventDestroy(t.a);
-      (void)hipEventDestroy(t.b);
+      give_timing(s, t.a);
+      give_timing(s, t.b);
     } else {
       s->pending_timing[keep++] = t;
     }
@@ -287,8 +318,8 @@ void collect_scatter(tile_stream_gpu* s)
   for (auto& t : s->pending_timing) {
     if (hipEventSynchronize(t.b) == hipSuccess)
       add_metric(s->metrics.*t.which, event_ms(t.a, t.b), t.in_bytes, t.out_bytes);
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    give_timing(s, t.a);
+    give_timing(s, t.b);
   }
   s->pending_timing.clear();
 }
@@ -379,10 +410,6 @@ int dispatch_stage(tile_stream_gpu* s)
   const int i = s->stage_cur;
   const uint64_t n = bytes / s->plan.bpe;
   const uint64_t E = epoch_elements(s);
-  hipEvent_t a, b;
-  HIPCHECK(hipEventCreate(&a));
-  HIPCHECK(hipEventCreate(&b));
-  HIPCHECK(hipEventRecord(a, s->st_h2d));
   if (s->plan.enable_multiscale) {
     if (linear_prepare(s))
       return 1;
@@ -404,12 +431,9 @@ int dispatch_stage(tile_stream_gpu* s)
       return 1;
   }
   HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
-  HIPCHECK(hipEventRecord(b, s->st_h2d));
   s->cursor += n;
   s->stage_fill = 0;
   s->stage_cur ^= 1;
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   s->metrics.h2d.input_bytes += (double)bytes;
   s->metrics.h2d.count++;
   return 0;
@@ -537,6 +561,16 @@ int flush_all(tile_stream_gpu* s)
 
 // ---- writer vtable ----
 
+// Joins and forgets a staging copy begun for input this call did not reach:
+// the caller's buffer is only valid during its append.
+void prefetch_drop(tile_stream_gpu* s)
+{
+  if (s->pf_n) {
+    s->copy_pool->wait();
+    s->pf_n = 0;
+  }
+}
+
 template<bool DEVICE>
 writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t* end)
 {
@@ -549,6 +583,7 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
   const uint64_t maxc = s->plan.max_cursor_elements;
   while (src < end) {
     if (maxc && s->cursor >= maxc) {
+      prefetch_drop(s);
       // capacity reached: everything is delivered once and the append says
       // finished; the caller's own flush still succeeds afterwards, as in
       // the reference (src/cpu/stream.c:545-548, 764-771)
@@ -593,8 +628,6 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
       uint64_t written = 0;
       while (written < bytes) {
         const int i = s->stage_cur;
-        if (s->stage_fill == 0 && hipEventSynchronize(s->ev_stage_free[i]) != hipSuccess)
-          return writer_error_at(src, end);
         const size_t payload = (size_t)std::min<uint64_t>(s->stage_cap - s->stage_fill, bytes - written);
         if (!s->copy_pool)
           s->copy_pool.reset(new (std::nothrow) CopyPool(default_copy_threads()));
@@ -603,7 +636,17 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
           return writer_error_at(src, end);
         }
         const auto t0 = std::chrono::steady_clock::now();
-        s->copy_pool->copy(s->h_stage[i] + s->stage_fill, src + written, payload);
+        if (s->pf_n && s->pf_src == src + written && s->pf_stage == i && s->stage_fill == 0 &&
+            s->pf_n == payload) {
+          // begun before the previous epoch's launches
+          s->copy_pool->wait();
+          s->pf_n = 0;
+        } else {
+          prefetch_drop(s);
+          if (s->stage_fill == 0 && hipEventSynchronize(s->ev_stage_free[i]) != hipSuccess)
+            return writer_error_at(src, end);
+          s->copy_pool->copy(s->h_stage[i] + s->stage_fill, src + written, payload);
+        }
         add_metric(s->metrics.memcpy,
                    std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count(),
                    (double)payload, (double)payload);
@@ -615,10 +658,28 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
       }
     }
     src += bytes;
-    if (s->cursor % E == 0 && s->cursor > 0)
-      if (end_epoch(s))
+    if (s->cursor % E == 0 && s->cursor > 0) {
+      // host appends: the next epoch's first staging copy runs on the copy
+      // threads while this thread issues the finished epoch's work
+      if (!DEVICE && s->copy_pool && s->stage_fill == 0) {
+        uint64_t ne = std::min<uint64_t>(E, (uint64_t)(end - src) / bpe);
+        if (maxc)
+          ne = std::min<uint64_t>(ne, maxc > s->cursor ? maxc - s->cursor : 0);
+        const size_t nb = (size_t)std::min<uint64_t>(ne * bpe, s->stage_cap);
+        if (nb && hipEventSynchronize(s->ev_stage_free[s->stage_cur]) == hipSuccess) {
+          s->copy_pool->start(s->h_stage[s->stage_cur], src, nb);
+          s->pf_src = src;
+          s->pf_n = nb;
+          s->pf_stage = s->stage_cur;
+        }
+      }
+      if (end_epoch(s)) {
+        prefetch_drop(s);
         return writer_error_at(src, end);
+      }
+    }
   }
+  prefetch_drop(s);
   return writer_result{ 0, { src, end } };
 }
 
@@ -654,6 +715,12 @@ void free_stream(tile_stream_gpu* s)
     (void)hipStreamSynchronize(s->st_d2h);
   if (s->ev_linear_free)
     (void)hipEventDestroy(s->ev_linear_free);
+  for (auto& t : s->pending_timing) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (hipEvent_t e : s->timing_free)
+    (void)hipEventDestroy(e);
   for (int i = 0; i < 2; ++i) {
     if (s->ev_h2d[i])
       (void)hipEventDestroy(s->ev_h2d[i]);
@@ -870,14 +937,15 @@ void metric_add(stream_metric& m, float ms, double in_bytes, double out_bytes)
 }
 void scatter_collect_ready(tile_stream_gpu* s) { collect_scatter_ready(s); }
 int timing_begin(tile_stream_gpu* s, hipEvent_t* a) { return time_begin(s, a); }
-int timing_pair(hipEvent_t* a, hipEvent_t* b)
+int timing_pair(tile_stream_gpu* s, hipEvent_t* a, hipEvent_t* b)
 {
-  *a = *b = nullptr;
-  if (hipEventCreate(a) != hipSuccess)
-    return 1;
-  if (hipEventCreate(b) != hipSuccess) {
-    (void)hipEventDestroy(*a);
-    *a = nullptr;
+  *a = take_timing(s);
+  *b = take_timing(s);
+  if (!*a || !*b) {
+    give_timing(s, *a);
+    give_timing(s, *b);
+    *a = *b = nullptr;
+    set_error("event creation failed");
     return 1;
   }
   return 0;

@@ -43,6 +43,10 @@ struct tile_stream_gpu
   size_t stage_fill = 0;
   int stage_cur = 0;
   std::unique_ptr<chucky::CopyPool> copy_pool; // staging copy threads, started at the first host append
+  // a staging copy begun before an epoch's launches: source, bytes, stage
+  const uint8_t* pf_src = nullptr;
+  size_t pf_n = 0;
+  int pf_stage = -1;
 
   // chunk pool: K epochs x total_chunks x chunk_stride elements
   uint8_t* d_pool = nullptr;
@@ -122,6 +126,7 @@ struct tile_stream_gpu
     stream_metric stream_metrics::*which;
   };
   std::vector<PendingTiming> pending_timing;
+  std::vector<hipEvent_t> timing_free; // recycled timing events
 
   chucky::BatchedState* batched = nullptr; // non-null: batched pipeline active
 
@@ -151,7 +156,7 @@ int timing_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*
 // Two timing events for launches that record them themselves (the start of
 // the first and the end of the last kernel of a pass), and their hand-over
 // to the same collection once those launches are queued.
-int timing_pair(hipEvent_t* a, hipEvent_t* b);
+int timing_pair(tile_stream_gpu* s, hipEvent_t* a, hipEvent_t* b);
 void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric stream_metrics::*which,
                 double in_bytes, double out_bytes);
 uint32_t compressor_slots();

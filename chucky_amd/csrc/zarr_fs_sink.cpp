@@ -550,6 +550,12 @@ int fs_write_direct(shard_writer* self, uint64_t offset, const void* beg, const 
 {
   FsWriter* w = reinterpret_cast<FsWriter*>(self);
   zarr_fs_sink* zs = w->owner;
+  // O_DIRECT has the kernel DMA from the caller's pages. The stream hands
+  // over pinned buffers from the HIP runtime, which are not ordinary user
+  // pages: O_DIRECT writes from them came out as zeros or failed, so
+  // unbuffered stores take the bounce copy into an aligned buffer.
+  if (zs->unbuffered)
+    return fs_write(self, offset, beg, end);
   const size_t n = (size_t)((const char*)end - (const char*)beg);
   if (n == 0)
     return 0;

@@ -157,8 +157,9 @@ def test_bounded_capacity_finishes(pipeline, device):
         (ge, gc), (re_, rc) = b.append(src)
     assert ge == re_ == 2  # writer_error_finished
     assert gc == rc == 96 * 2
-    # the stream flushed itself at capacity; a later flush or append says so
-    assert b.flush() == (2, 2)
+    # the stream delivered everything at capacity; the caller's flush then
+    # succeeds (src/cpu/stream.c:545-548, 764-771) and later appends say finished
+    assert b.flush() == (0, 0)
     (ge, gc), (re_, rc) = b.append(src[:4])
     assert ge == re_ == 2 and gc == rc == 0
     gs = b.check_same()
