@@ -188,6 +188,8 @@ _sig("chucky_discard_sink_finalized", C.c_uint64, _P)
 _sig("chucky_gpu_max_output_size", C.c_size_t, C.c_int, C.c_size_t)
 _sig("chucky_gpu_compress", C.c_int, C.c_int, C.c_int, _P, C.c_size_t, C.c_size_t,
      C.c_uint64, _P, C.c_size_t, _P)
+_sig("chucky_gpu_compress_noalloc", C.c_int, C.c_int, C.c_int, _P, C.c_size_t, C.c_size_t,
+     C.c_uint64, _P, C.c_size_t, _P, _P)
 _sig("chucky_gpu_transpose", C.c_int, _P, _P, C.c_uint64, C.c_uint8, C.c_uint64, C.c_uint8,
      C.POINTER(C.c_uint64), C.POINTER(C.c_int64))
 _sig("chucky_crc32c", C.c_uint32, _P, C.c_size_t)

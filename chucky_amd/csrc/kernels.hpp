@@ -121,6 +121,16 @@ hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_b
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
                        uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
                        void* tables, hipStream_t stream);
+// launch_zstd with the predefined FSE tables at `predefined` (device memory
+// holding zstd_predefined_bytes() bytes written by zstd_build_predefined)
+// instead of the per-device copy zstd_prepare() allocates.
+hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t src_stride,
+                           uint64_t chunk_bytes, uint64_t n, uint8_t* dst, uint64_t chunk_dst_stride,
+                           uint32_t* seg_sizes, int level, uint32_t block, void* scratch, uint32_t workers,
+                           uint32_t* queue_counter, void* tables, hipStream_t stream);
+size_t zstd_predefined_bytes();
+// Writes the predefined FSE tables into device memory at dst (one thread).
+hipError_t zstd_build_predefined(void* dst, hipStream_t stream);
 // Chunk-table mode (tables != nullptr, zstd_table_bytes(n) of device memory):
 // one Huffman table per chunk, built from the chunk's byte histogram by a
 // workgroup per chunk, carried by a small first block of the frame and reused

@@ -1483,19 +1483,50 @@ size_t zstd_scratch_bytes(uint32_t workers) { return (size_t)workers * kWorkerSc
 
 size_t zstd_table_bytes(uint64_t n) { return (size_t)n * sizeof(ChunkTable); }
 
+size_t zstd_predefined_bytes() { return sizeof(zcore::Predefined); }
+
+namespace {
+__global__ void predefined_k(zcore::Predefined* out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    uint8_t spread[512];
+    uint32_t cumul[66];
+    zcore::build_predefined(*out, spread, cumul);
+  }
+}
+} // namespace
+
+hipError_t zstd_build_predefined(void* dst, hipStream_t stream)
+{
+  hipLaunchKernelGGL(predefined_k, dim3(1), dim3(1), 0, stream, (zcore::Predefined*)dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_zstd(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                        uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes, int level,
                        uint32_t block, void* scratch, uint32_t workers, uint32_t* queue_counter,
                        void* tables, hipStream_t stream)
+{
+  const zcore::Predefined* pre = predefined_on_current_device();
+  if (!pre && n)
+    return hipErrorNotInitialized; // zstd_prepare() was not called on this device
+  return launch_zstd_pre(pre, src, src_stride, chunk_bytes, n, dst, chunk_dst_stride, seg_sizes, level,
+                         block, scratch, workers, queue_counter, tables, stream);
+}
+
+hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t src_stride,
+                           uint64_t chunk_bytes, uint64_t n, uint8_t* dst, uint64_t chunk_dst_stride,
+                           uint32_t* seg_sizes, int level, uint32_t block, void* scratch, uint32_t workers,
+                           uint32_t* queue_counter, void* tables, hipStream_t stream)
 {
   (void)level;
   if (n == 0)
     return hipSuccess;
   if (chunk_bytes == 0 || chunk_bytes > 0xFFFFFFFFull || !zstd_block_ok(block))
     return hipErrorInvalidValue;
-  const zcore::Predefined* pre = predefined_on_current_device();
+  const zcore::Predefined* pre = (const zcore::Predefined*)predefined;
   if (!pre)
-    return hipErrorNotInitialized; // zstd_prepare() was not called on this device
+    return hipErrorInvalidValue;
   const uint32_t nseg = zstd_segments(chunk_bytes, block);
   const uint64_t total = n * nseg;
   ChunkTable* tabs = (ChunkTable*)tables;

@@ -164,6 +164,21 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
                         size_t src_stride, size_t chunk_bytes, uint64_t n, void* d_dst,
                         size_t dst_stride, uint64_t* d_sizes);
 
+/* No counterpart in the reference: chucky_gpu_compress for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops). The
+ * same inputs, outputs and compressed bytes, with every other buffer in one
+ * file-scope device array of the library (about 25 MiB a device, held until
+ * the process ends): no hipMalloc, hipFree or stream creation. Asynchronous
+ * on `stream` (a hipStream_t; NULL = the null stream): synchronize it before
+ * reading d_dst / d_sizes. At most 65536 chunks x zstd blocks of 128 KiB per
+ * call (lz4: 65536 chunks); larger calls fail with an error and launch
+ * nothing. The first call on a device prepares that device's tables and must
+ * not be under graph capture. Every call on a device shares the array, so
+ * calls that overlap on one device race. */
+int chucky_gpu_compress_noalloc(enum compression_codec codec, int level, const void* d_src,
+                                size_t src_stride, size_t chunk_bytes, uint64_t n, void* d_dst,
+                                size_t dst_stride, uint64_t* d_sizes, void* stream);
+
 /* Scatter n elements (element index i_offset + i of the lifted row-major space)
  * into a chunk pool: dst[ravel(shape, strides, i_offset+i)] = src[i]. */
 int chucky_gpu_transpose(void* d_dst, const void* d_src, uint64_t n, uint8_t bpe,
