@@ -63,30 +63,13 @@ int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_s
     set_error("chucky_gpu_compress: unsupported codec");
     goto done;
   }
-  if (hipStreamSynchronize(st) != hipSuccess)
+  // per-chunk frame sizes; zstd segments moved behind segment 0 on the
+  // device (the moves overlap, which a copy call does not allow)
+  if (launch_frame_sizes((uint8_t*)d_dst, dst_stride, seg, nseg,
+                         codec == CODEC_ZSTD ? zstd_segment_stride(kZstdBlockMax) : 0, n, d_sizes, st) !=
+        hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
     goto done;
-  {
-    // Per-chunk compressed bytes. For zstd the segments of a chunk sit at a
-    // fixed stride inside its slot; compact them so each chunk's frame is
-    // contiguous at d_dst + i*dst_stride.
-    std::basic_string<uint32_t> h(n * nseg, 0);
-    if (hipMemcpy(&h[0], seg, n * nseg * 4, hipMemcpyDeviceToHost) != hipSuccess)
-      goto done;
-    std::basic_string<uint64_t> sizes(n, 0);
-    for (uint64_t i = 0; i < n; ++i) {
-      uint64_t off = h[i * nseg];
-      for (uint32_t k = 1; k < nseg; ++k) {
-        uint8_t* base = (uint8_t*)d_dst + i * dst_stride;
-        if (hipMemcpy(base + off, base + (uint64_t)k * zstd_segment_stride(kZstdBlockMax), h[i * nseg + k],
-                      hipMemcpyDeviceToDevice) != hipSuccess)
-          goto done;
-        off += h[i * nseg + k];
-      }
-      sizes[i] = off;
-    }
-    if (hipMemcpy(d_sizes, sizes.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
-      goto done;
-  }
   rc = 0;
 done:
   if (rc && chucky_last_error()[0] == 0)

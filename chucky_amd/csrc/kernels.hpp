@@ -129,6 +129,12 @@ hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t 
                            uint32_t* seg_sizes, int level, uint32_t block, void* scratch, uint32_t workers,
                            uint32_t* queue_counter, void* tables, hipStream_t stream);
 size_t zstd_predefined_bytes();
+// Per-chunk frame sizes into sizes[0, n) from the segment sizes of a batch
+// compress; with nseg > 1 (zstd blocks at seg_stride) segments 1.. of each
+// chunk are first moved down behind segment 0, so every frame is contiguous
+// at dst + i*dst_stride.
+hipError_t launch_frame_sizes(uint8_t* dst, uint64_t dst_stride, const uint32_t* seg, uint32_t nseg,
+                              uint64_t seg_stride, uint64_t n, uint64_t* sizes, hipStream_t stream);
 // Writes the predefined FSE tables into device memory at dst (one thread).
 hipError_t zstd_build_predefined(void* dst, hipStream_t stream);
 // Chunk-table mode (tables != nullptr, zstd_table_bytes(n) of device memory):

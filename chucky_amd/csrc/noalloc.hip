@@ -47,48 +47,6 @@ __global__ void zero_k(uint32_t* p, uint32_t n)
     p[i] = 0;
 }
 
-// Moves segments 1.. of every chunk's zstd frame down behind segment 0 and
-// writes the frame's size. Destinations lie below their sources, and each
-// round reads all of its bytes before writing any, so the moves never
-// overwrite bytes still to be read.
-__global__ __launch_bounds__(256) void compact_k(uint8_t* dst, uint64_t dst_stride, const uint32_t* seg,
-                                                 uint32_t nseg, uint64_t seg_stride, uint64_t* sizes)
-{
-  const uint64_t c = blockIdx.x;
-  uint8_t* base = dst + c * dst_stride;
-  const uint32_t* s = seg + c * nseg;
-  uint64_t off = s[0];
-  for (uint32_t k = 1; k < nseg; ++k) {
-    const uint32_t len = s[k];
-    const uint8_t* from = base + (uint64_t)k * seg_stride;
-    uint8_t* to = base + off;
-    for (uint32_t r = 0; r < len; r += 256 * 16) {
-      uint8_t v[16];
-      const uint32_t b = r + threadIdx.x * 16;
-#pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (b + j < len)
-          v[j] = from[b + j];
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (b + j < len)
-          to[b + j] = v[j];
-      __syncthreads();
-    }
-    off += len;
-  }
-  if (threadIdx.x == 0)
-    sizes[c] = off;
-}
-
-__global__ __launch_bounds__(256) void widen_k(const uint32_t* seg, uint64_t* sizes, uint64_t n)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n)
-    sizes[i] = seg[i];
-}
-
 // The device address of `scratch` on the current device, found (and the
 // zstd tables and work counter prepared) at the first call on each device.
 uint8_t* scratch_on_device(hipStream_t st)
@@ -132,6 +90,69 @@ uint8_t* scratch_on_device(hipStream_t st)
 
 } // namespace
 
+
+namespace chucky {
+
+namespace {
+// Moves segments 1.. of every chunk's zstd frame down behind segment 0 and
+// writes the frame's size. Destinations lie below their sources, and each
+// round reads all of its bytes before writing any, so the moves never
+// overwrite bytes still to be read.
+__global__ __launch_bounds__(256) void compact_k(uint8_t* dst, uint64_t dst_stride, const uint32_t* seg,
+                                                 uint32_t nseg, uint64_t seg_stride, uint64_t* sizes)
+{
+  const uint64_t c = blockIdx.x;
+  uint8_t* base = dst + c * dst_stride;
+  const uint32_t* s = seg + c * nseg;
+  uint64_t off = s[0];
+  for (uint32_t k = 1; k < nseg; ++k) {
+    const uint32_t len = s[k];
+    const uint8_t* from = base + (uint64_t)k * seg_stride;
+    uint8_t* to = base + off;
+    for (uint32_t r = 0; r < len; r += 256 * 16) {
+      uint8_t v[16];
+      const uint32_t b = r + threadIdx.x * 16;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (b + j < len)
+          v[j] = from[b + j];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (b + j < len)
+          to[b + j] = v[j];
+      __syncthreads();
+    }
+    off += len;
+  }
+  if (threadIdx.x == 0)
+    sizes[c] = off;
+}
+
+__global__ __launch_bounds__(256) void widen_k(const uint32_t* seg, uint64_t* sizes, uint64_t n)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    sizes[i] = seg[i];
+}
+
+} // namespace
+
+hipError_t launch_frame_sizes(uint8_t* dst, uint64_t dst_stride, const uint32_t* seg, uint32_t nseg,
+                              uint64_t seg_stride, uint64_t n, uint64_t* sizes, hipStream_t stream)
+{
+  if (n == 0)
+    return hipSuccess;
+  if (nseg == 1)
+    hipLaunchKernelGGL(widen_k, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, seg, sizes, n);
+  else
+    hipLaunchKernelGGL(compact_k, dim3((uint32_t)n), dim3(256), 0, stream, dst, dst_stride, seg, nseg,
+                       seg_stride, sizes);
+  return hipGetLastError();
+}
+
+} // namespace chucky
+
 extern "C" {
 
 int chucky_gpu_compress_noalloc(enum compression_codec codec, int level, const void* d_src,
@@ -164,20 +185,16 @@ int chucky_gpu_compress_noalloc(enum compression_codec codec, int level, const v
   if (codec == CODEC_LZ4_NON_STANDARD) {
     e = launch_lz4((const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride, seg, 12, 0,
                    nullptr, st);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(widen_k, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, seg, d_sizes, n);
-      e = hipGetLastError();
-    }
+    if (e == hipSuccess)
+      e = launch_frame_sizes((uint8_t*)d_dst, dst_stride, seg, 1, 0, n, d_sizes, st);
   } else {
     const uint32_t workers = (uint32_t)std::min<uint64_t>(n * nseg, kWorkers);
     e = launch_zstd_pre(ptr, (const uint8_t*)d_src, src_stride, chunk_bytes, n, (uint8_t*)d_dst, dst_stride,
                         seg, level, kZstdBlockMax, ptr + kWorkOff, workers, (uint32_t*)(ptr + kQueueOff),
                         nullptr, st);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(compact_k, dim3((uint32_t)n), dim3(256), 0, st, (uint8_t*)d_dst, (uint64_t)dst_stride,
-                         seg, nseg, zstd_segment_stride(kZstdBlockMax), d_sizes);
-      e = hipGetLastError();
-    }
+    if (e == hipSuccess)
+      e = launch_frame_sizes((uint8_t*)d_dst, dst_stride, seg, nseg, zstd_segment_stride(kZstdBlockMax), n,
+                             d_sizes, st);
   }
   if (e != hipSuccess) {
     set_error(std::string("chucky_gpu_compress_noalloc: launch failed: ") + hipGetErrorString(e));

@@ -50,10 +50,13 @@ def test_noalloc_matches_the_allocating_entry_point(codec):
         print(f"[noalloc] codec={name} chunks={n} chunk_bytes={chunk}", flush=True)
         want_sz, want = _run(ca.lib.chucky_gpu_compress, codec, host, chunk, n, False)
         got_sz, got = _run(ca.lib.chucky_gpu_compress_noalloc, codec, host, chunk, n, True)
+        for i in range(n):
+            src = host[i * chunk:(i + 1) * chunk].tobytes()
+            assert decode(name, want[i], chunk) == src, f"allocating entry point, chunk {i}"
+            assert decode(name, got[i], chunk) == src, f"no-alloc entry point, chunk {i}"
         np.testing.assert_array_equal(got_sz, want_sz)
         for i in range(n):
             assert got[i] == want[i], f"chunk {i} differs"
-            assert decode(name, got[i], chunk) == host[i * chunk:(i + 1) * chunk].tobytes(), i
 
 
 def test_noalloc_refuses_calls_past_its_limits():
