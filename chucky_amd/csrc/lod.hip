@@ -969,6 +969,10 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
                            s->d_gather_fixed, n0, p.fixed_count, st, ta, tb) != hipSuccess)
     return 1;
   timing_add(s, ta, tb, &stream_metrics::lod_gather, nb * n0 * bpe, nb * n0 * bpe);
+  // the epoch buffer is free for the next epoch's copy once the gather has
+  // read it; the rest of the pass works on the Morton levels
+  if (hipEventRecord(s->ev_linear_free, st) != hipSuccess)
+    return 1;
   int last_reduce = -1;
   for (int l = 0; l + 1 < p.nlod; ++l)
     if (p.levels[l + 1].lod_nelem)

@@ -477,9 +477,9 @@ int end_epoch(tile_stream_gpu* s)
 {
   uint32_t mask = 1;
   if (s->plan.enable_multiscale) {
+    // (the LOD pass records ev_linear_free once its gather has read the epoch)
     if (lod_run_epoch(s, s->batch_acc, &mask))
       return 1;
-    HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
   }
   if (s->batched) {
     s->epochs_done++;
