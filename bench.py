@@ -407,6 +407,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads beside the headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--zstd-block", type=int, default=0, help="zstd block bytes (0 = autotuned)")
+    ap.add_argument("--zstd-chunk-tables", type=int, default=0,
+                    help="zstd Huffman tables: 0 = autotuned, 1 = one per chunk, 2 = one per block")
     ap.add_argument("--batched-launch", type=int, default=0,
                     help="batched pipeline per-epoch work: 0 = autotuned, 1 = captured graphs, 2 = direct")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -427,7 +429,8 @@ def main():
 
     import chucky_amd as ca
     host_input = not args.device_input
-    options = {"zstd_block_bytes": args.zstd_block, "batched_launch": args.batched_launch}
+    options = {"zstd_block_bytes": args.zstd_block, "batched_launch": args.batched_launch,
+               "zstd_chunk_tables": args.zstd_chunk_tables}
     cfg, shape, desc = workload(args.workload, args.frames)
     data, host = make_input(args.workload, shape, True)
     h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)
