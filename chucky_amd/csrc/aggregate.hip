@@ -19,6 +19,14 @@ namespace chucky {
 namespace {
 
 constexpr int kPlanThreads = 1024;
+constexpr uint64_t kGatherPiece = 64 * 1024; // bytes one workgroup copies at most
+
+uint32_t pieces_of(const SegmentSource& s)
+{
+  const uint64_t seg = s.seg_sizes ? s.seg_stride : s.const_size;
+  const uint64_t p = (seg + kGatherPiece - 1) / kGatherPiece;
+  return (uint32_t)(p ? p : 1);
+}
 
 __device__ __forceinline__ uint64_t item_bytes(const SegmentSource& s, uint32_t chunk)
 {
@@ -139,26 +147,30 @@ __device__ __forceinline__ void block_copy(uint8_t* out, const uint8_t* in, uint
     out[tail0 + threadIdx.x] = in[tail0 + threadIdx.x];
 }
 
-// Workgroups take (item, segment) pairs with a grid stride: a compressed
-// segment lands at its chunk's offset plus the sizes of the chunk's earlier
-// segments (summed by the workgroup), so a batch of few large chunks still
-// spreads over the whole GPU. Constant-size sources (CODEC_NONE chunks split
-// into fixed pieces) are walked as (item, piece) pairs the same way.
+// Workgroups take (item, segment, piece) triples with a grid stride: a
+// compressed segment lands at its chunk's offset plus the sizes of the
+// chunk's earlier segments (summed by the workgroup), and a segment larger
+// than kGatherPiece is copied by several workgroups, so a batch of few large
+// chunks (one LZ4 segment per chunk) still spreads over the whole GPU.
+// Constant-size sources (CODEC_NONE chunks split into fixed segments) are
+// walked the same way.
 // A gather into pinned host memory runs at PCIe speed, so it is given only a
 // few CUs and leaves the rest to the compressor and transpose beside it.
 // One (item, segment) pair of a level (uniform across the workgroup).
 __device__ __forceinline__ void gather_pair(const SegmentSource& src, const uint32_t* __restrict__ gather,
                                             const uint32_t* __restrict__ perm,
                                             const uint64_t* __restrict__ offsets, uint8_t* __restrict__ dst,
-                                            uint64_t pr, uint32_t* part)
+                                            uint64_t tr, uint32_t* part)
 {
+  const uint64_t pr = tr / src.pieces;
+  const uint64_t piece_off = (uint64_t)(uint32_t)(tr - pr * src.pieces) * kGatherPiece;
   const uint64_t item = pr / src.nseg;
   const uint32_t seg = (uint32_t)(pr - item * src.nseg);
   const uint32_t chunk = gather[item];
   if (src.seg_sizes) {
     const uint32_t* sz = src.seg_sizes + (uint64_t)chunk * src.nseg;
     const uint32_t nb = sz[seg];
-    if (nb == 0)
+    if (nb <= piece_off)
       return;  // uniform across the workgroup: no barrier is skipped by part of it
     // a chunk's compressed bytes (below its slot stride) fit 32 bits
     uint32_t before = 0;
@@ -170,16 +182,18 @@ __device__ __forceinline__ void gather_pair(const SegmentSource& src, const uint
     __syncthreads();
     before = part[0] + part[1] + part[2] + part[3];
     __syncthreads();
-    block_copy(dst + offsets[perm[item]] + before,
-               src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nb);
+    const uint64_t n = nb - piece_off < kGatherPiece ? nb - piece_off : kGatherPiece;
+    block_copy(dst + offsets[perm[item]] + before + piece_off,
+               src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride + piece_off, n);
     return;
   }
   const uint64_t seg_off = (uint64_t)seg * src.const_size;
   const uint64_t nbytes = seg + 1 == src.nseg ? src.const_total - seg_off : src.const_size;
-  if (nbytes == 0 || seg_off >= src.const_total)
+  if (nbytes <= piece_off || seg_off >= src.const_total)
     return;
-  block_copy(dst + offsets[perm[item]] + seg_off,
-             src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride, nbytes);
+  const uint64_t n = nbytes - piece_off < kGatherPiece ? nbytes - piece_off : kGatherPiece;
+  block_copy(dst + offsets[perm[item]] + seg_off + piece_off,
+             src.base + (uint64_t)chunk * src.chunk_stride + (uint64_t)seg * src.seg_stride + piece_off, n);
 }
 
 __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
@@ -190,7 +204,7 @@ __global__ __launch_bounds__(256) void aggregate_gather_k(SegmentSource src,
                                                           uint8_t* __restrict__ dst)
 {
   __shared__ uint32_t part[256 / 64];
-  const uint64_t pairs = n_items * src.nseg;
+  const uint64_t pairs = n_items * src.nseg * src.pieces;
   for (uint64_t pr = blockIdx.x; pr < pairs; pr += gridDim.x)
     gather_pair(src, gather, perm, offsets, dst, pr, part);
 }
@@ -201,7 +215,7 @@ __global__ __launch_bounds__(256) void aggregate_gather_levels_k(SegmentSource s
   __shared__ uint32_t part[256 / 64];
   uint64_t first = 0, base = 0;
   for (int k = 0; k < a.nlev; ++k) {
-    const uint64_t pairs = a.n_items[k] * src.nseg;
+    const uint64_t pairs = a.n_items[k] * src.nseg * src.pieces;
     // this workgroup's first pair of level k under the grid stride
     uint64_t pr = first <= blockIdx.x ? blockIdx.x - first
                                       : (gridDim.x - (first - blockIdx.x) % gridDim.x) % gridDim.x;
@@ -232,15 +246,17 @@ hipError_t launch_aggregate_levels(const SegmentSource& src, const AggLevels& a,
 {
   if (a.nlev <= 0 || a.nlev > kAggMaxLevels)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(aggregate_plan_levels_k, dim3((uint32_t)a.nlev), dim3(kPlanThreads), 0, stream, src, a);
+  SegmentSource s = src;
+  s.pieces = pieces_of(src);
+  hipLaunchKernelGGL(aggregate_plan_levels_k, dim3((uint32_t)a.nlev), dim3(kPlanThreads), 0, stream, s, a);
   uint64_t pairs = 0;
   for (int k = 0; k < a.nlev; ++k)
-    pairs += a.n_items[k] * src.nseg;
+    pairs += a.n_items[k] * s.nseg * s.pieces;
   if (pairs == 0)
     return hipGetLastError();
   const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
   const uint64_t wg = pairs < cap ? pairs : cap;
-  hipLaunchKernelGGL(aggregate_gather_levels_k, dim3((uint32_t)wg), dim3(256), 0, stream, src, a);
+  hipLaunchKernelGGL(aggregate_gather_levels_k, dim3((uint32_t)wg), dim3(256), 0, stream, s, a);
   return hipGetLastError();
 }
 
@@ -250,11 +266,13 @@ hipError_t launch_aggregate_gather(const SegmentSource& src, const uint32_t* gat
 {
   if (n_items == 0)
     return hipSuccess;
-  uint64_t wg = n_items * src.nseg;
+  SegmentSource s = src;
+  s.pieces = pieces_of(src);
+  uint64_t wg = n_items * s.nseg * s.pieces;
   const uint64_t cap = max_workgroups ? max_workgroups : (1u << 20);
   if (wg > cap)
     wg = cap;
-  hipLaunchKernelGGL(aggregate_gather_k, dim3((uint32_t)wg), dim3(256), 0, stream, src, gather, perm,
+  hipLaunchKernelGGL(aggregate_gather_k, dim3((uint32_t)wg), dim3(256), 0, stream, s, gather, perm,
                      n_items, offsets, dst);
   return hipGetLastError();
 }

@@ -37,6 +37,7 @@ struct SegmentSource
   const uint32_t* seg_sizes;
   uint64_t const_size;      // used when seg_sizes == nullptr (last segment gets the remainder)
   uint64_t const_total;     // chunk bytes when seg_sizes == nullptr
+  uint32_t pieces;          // set by the gather launchers: pieces a segment is copied in
 };
 
 // One workgroup: chunk_sizes[perm[i]] = size(gather[i]) for i < n_items,

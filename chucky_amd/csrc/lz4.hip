@@ -16,6 +16,8 @@
 // matches start at most 12 bytes before the end and end at least 5 bytes
 // before it; offsets are at most 65535.
 #include "kernels.hpp"
+
+#include <algorithm>
 #include "wave.hpp"
 
 namespace chucky {
@@ -272,50 +274,84 @@ __device__ __forceinline__ void block_copy(uint8_t* dst, const uint8_t* src, uin
     dst[t0 + t] = src[t0 + t];
 }
 
-__device__ __forceinline__ uint32_t put_token(uint8_t* out, uint32_t op, uint32_t lit, uint32_t nib)
+// Where the stitched block puts one segment's pieces: its literal run (input
+// bytes [lit_src, lit_src + lit_len), carried from earlier segments without
+// a match) at lit_dst, its sequences (scratch) at body_dst. Entry nseg of a
+// chunk is the block's closing literal run (body_len 0).
+struct SegPlace
 {
-  if (threadIdx.x == 0) {
-    out[op] = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | nib);
-    if (lit >= 15)
-      put_len_ext(out, op + 1, lit);
-  }
-  return op + 1 + (lit >= 15 ? 1 + (lit - 15) / 255 : 0);
-}
+  uint32_t lit_src, lit_len, lit_dst, body_dst, body_len;
+};
 
-__global__ __launch_bounds__(256) void lz4_stitch_k(const uint8_t* __restrict__ src,
-                                                    uint64_t src_stride, uint32_t n,
-                                                    uint32_t seg_len, uint32_t nseg,
-                                                    uint64_t nchunks, const uint8_t* __restrict__ tmp,
-                                                    uint64_t seg_bound, const SegRec* __restrict__ recs,
-                                                    uint8_t* __restrict__ dst, uint64_t dst_stride,
-                                                    uint32_t* __restrict__ sizes)
+// One thread per chunk: walks the chunk's segment records in order, writes
+// each literal run's token (with the first match's length nibble) and length
+// extension into the block, and records where the literal and sequence bytes
+// go; the closing literal-only sequence ends the block (LZ4 end rules).
+__global__ __launch_bounds__(64) void lz4_layout_k(uint32_t n, uint32_t seg_len, uint32_t nseg,
+                                                   uint64_t nchunks, const SegRec* __restrict__ recs,
+                                                   SegPlace* __restrict__ places, uint8_t* __restrict__ dst,
+                                                   uint64_t dst_stride, uint32_t* __restrict__ sizes)
 {
-  const uint64_t chunk = blockIdx.x;
+  const uint64_t chunk = (uint64_t)blockIdx.x * 64 + threadIdx.x;
   if (chunk >= nchunks)
     return;
-  const uint8_t* in = src + chunk * src_stride;
   uint8_t* out = dst + chunk * dst_stride;
+  SegPlace* pl = places + chunk * (nseg + 1);
   uint32_t op = 0, carry = 0; // carry: input position where pending literals start
   for (uint32_t seg = 0; seg < nseg; ++seg) {
     const SegRec r = recs[chunk * nseg + seg];
-    if (r.nseq == 0)
+    if (r.nseq == 0) {
+      pl[seg] = SegPlace{ 0, 0, 0, 0, 0 };
       continue;
+    }
     const uint32_t s0 = seg * seg_len;
     const uint32_t lit = s0 + r.ll0 - carry;
-    op = put_token(out, op, lit, r.tok0);
-    block_copy(out + op, in + carry, lit);
-    op += lit;
-    block_copy(out + op, tmp + (chunk * nseg + seg) * seg_bound, r.body);
-    op += r.body;
+    out[op] = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | r.tok0);
+    op = lit >= 15 ? put_len_ext(out, op + 1, lit) : op + 1;
+    pl[seg] = SegPlace{ carry, lit, op, op + lit, r.body };
+    op += lit + r.body;
     const uint32_t s1 = n - s0 < seg_len ? n : s0 + seg_len;
     carry = s1 - r.tail;
   }
   const uint32_t lit = n - carry;
-  op = put_token(out, op, lit, 0);
-  block_copy(out + op, in + carry, lit);
-  op += lit;
-  if (threadIdx.x == 0)
-    sizes[chunk] = op;
+  out[op] = (uint8_t)((lit >= 15 ? 15 : lit) << 4);
+  op = lit >= 15 ? put_len_ext(out, op + 1, lit) : op + 1;
+  pl[nseg] = SegPlace{ carry, lit, op, op + lit, 0 };
+  sizes[chunk] = op + lit;
+}
+
+constexpr uint32_t kStitchPiece = 16 * 1024;
+
+// Grid-stride over (chunk, segment, part, piece): part 0 copies a piece of
+// the segment's literal run from the input, part 1 a piece of its sequences
+// from scratch, each to its place in the block.
+__global__ __launch_bounds__(256) void lz4_copy_k(const uint8_t* __restrict__ src, uint64_t src_stride,
+                                                  uint32_t nseg, uint64_t nchunks, uint32_t lit_pieces,
+                                                  uint32_t body_pieces, const uint8_t* __restrict__ tmp,
+                                                  uint64_t seg_bound, const SegPlace* __restrict__ places,
+                                                  uint8_t* __restrict__ dst, uint64_t dst_stride)
+{
+  const uint64_t per_seg = (uint64_t)lit_pieces + body_pieces;
+  const uint64_t total = nchunks * (nseg + 1) * per_seg;
+  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint64_t cs = w / per_seg; // chunk * (nseg + 1) + seg
+    const uint32_t k = (uint32_t)(w - cs * per_seg);
+    const uint64_t chunk = cs / (nseg + 1);
+    const uint32_t seg = (uint32_t)(cs - chunk * (nseg + 1));
+    const SegPlace pl = places[cs];
+    uint8_t* out = dst + chunk * dst_stride;
+    if (k < lit_pieces) {
+      const uint32_t off = k * kStitchPiece;
+      if (off < pl.lit_len)
+        block_copy(out + pl.lit_dst + off, src + chunk * src_stride + pl.lit_src + off,
+                   pl.lit_len - off < kStitchPiece ? pl.lit_len - off : kStitchPiece);
+    } else {
+      const uint32_t off = (k - lit_pieces) * kStitchPiece;
+      if (off < pl.body_len)
+        block_copy(out + pl.body_dst + off, tmp + (chunk * nseg + seg) * seg_bound + off,
+                   pl.body_len - off < kStitchPiece ? pl.body_len - off : kStitchPiece);
+    }
+  }
 }
 
 } // namespace
@@ -333,7 +369,7 @@ size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes)
   if (!seg_bytes || seg_bytes >= chunk_bytes)
     return 0;
   const uint64_t nseg = (chunk_bytes + seg_bytes - 1) / seg_bytes;
-  return n * nseg * (lz4_segment_bound(seg_bytes) + sizeof(SegRec));
+  return n * nseg * (lz4_segment_bound(seg_bytes) + sizeof(SegRec)) + n * (nseg + 1) * sizeof(SegPlace) + 256;
 }
 
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
@@ -378,9 +414,14 @@ hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_by
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
-  hipLaunchKernelGGL(lz4_stitch_k, dim3((uint32_t)n), dim3(256), 0, stream, src, src_stride,
-                     (uint32_t)chunk_bytes, seg_bytes, nseg, n, tmp, bound, recs, dst, dst_stride,
-                     sizes);
+  SegPlace* places = (SegPlace*)(((uintptr_t)(recs + total) + 15) & ~(uintptr_t)15);
+  hipLaunchKernelGGL(lz4_layout_k, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, stream, (uint32_t)chunk_bytes,
+                     seg_bytes, nseg, n, recs, places, dst, dst_stride, sizes);
+  const uint32_t lit_pieces = (uint32_t)((chunk_bytes + kStitchPiece - 1) / kStitchPiece);
+  const uint32_t body_pieces = (uint32_t)((bound + kStitchPiece - 1) / kStitchPiece);
+  const uint64_t items = n * (nseg + 1) * ((uint64_t)lit_pieces + body_pieces);
+  hipLaunchKernelGGL(lz4_copy_k, dim3((uint32_t)std::min<uint64_t>(items, 1u << 16)), dim3(256), 0, stream,
+                     src, src_stride, nseg, n, lit_pieces, body_pieces, tmp, bound, places, dst, dst_stride);
   return hipGetLastError();
 }
 
