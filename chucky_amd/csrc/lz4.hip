@@ -26,6 +26,9 @@ namespace {
 constexpr uint32_t kMaxOffset = 65535;
 constexpr uint32_t kMfLimit = 12;
 constexpr uint32_t kLastLiterals = 5;
+// LDS staging capacities of the segment parser (bytes of input per wavefront)
+constexpr int kLz4StageSmall = 8 * 1024;
+constexpr int kLz4StageMax = 16 * 1024;
 
 __device__ __forceinline__ uint32_t lz4_hash(uint32_t v, int hash_log)
 {
@@ -173,7 +176,13 @@ __device__ __forceinline__ uint32_t emit_match_part(uint8_t* out, uint32_t op, u
   return op + 2 + (ml4 >= 15 ? 1 + (ml4 - 15) / 255 : 0);
 }
 
-template<int HASH_LOG>
+// One wavefront per segment of a chunk. With STAGE > 0 (segments of at most
+// STAGE bytes) the segment and the 8 bytes after it are first copied into
+// LDS with wide loads issued together, so the parse loop's position and
+// candidate loads and the match extension read LDS instead of waiting on
+// HBM latency step after step; only the backward extension, which may reach
+// before the segment, still reads the chunk in global memory.
+template<int HASH_LOG, int STAGE>
 __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ src,
                                                     uint64_t src_stride, uint32_t n,
                                                     uint32_t seg_len, uint32_t nseg,
@@ -181,6 +190,7 @@ __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ 
                                                     uint64_t seg_bound, SegRec* __restrict__ recs)
 {
   __shared__ uint32_t table[1 << HASH_LOG]; // position + 1, 0 = empty
+  __shared__ uint4 stage_v[STAGE ? (STAGE + 16) / 16 : 1];
   const uint64_t item = blockIdx.x;
   if (item >= total)
     return;
@@ -193,6 +203,21 @@ __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ 
   const uint32_t s1 = n - s0 < seg_len ? n : s0 + seg_len;
   for (int i = lane; i < (1 << HASH_LOG); i += kWave)
     table[i] = 0;
+  const uint8_t* base = in + s0; // input byte s0 + i is base[i] for i < s1 - s0 + 8
+  if constexpr (STAGE > 0) {
+    const uint32_t nb = s1 - s0 + 8; // <= STAGE + 8: fits stage_v rounded up to 16
+    const uint8_t* g = in + s0;
+    if (((uintptr_t)g & 15) == 0) {
+      const uint4* gv = (const uint4*)g;
+      for (uint32_t i = lane; i < (nb + 15) / 16; i += kWave)
+        stage_v[i] = gv[i];
+    } else {
+      uint32_t* sw = (uint32_t*)stage_v;
+      for (uint32_t i = lane; i < (nb + 3) / 4; i += kWave)
+        sw[i] = ld32u(g + 4 * i);
+    }
+    base = (const uint8_t*)stage_v;
+  }
   __syncthreads();
 
   uint32_t op = 0, anchor = s0, nseq = 0, ll0 = 0, tok0 = 0;
@@ -207,14 +232,15 @@ __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ 
       const bool active = p <= last_start;
       uint32_t v = 0, h = 0, cand = 0;
       if (active) {
-        v = ld32u(in + p);
+        v = ld32u(base + (p - s0));
         h = lz4_hash(v, HASH_LOG);
         cand = table[h];
       }
+      // table entries are positions of this segment: c >= s0
       bool hit = false;
       if (active && cand != 0) {
         const uint32_t c = cand - 1;
-        hit = c < p && (p - c) <= kMaxOffset && ld32u(in + c) == v;
+        hit = c < p && (p - c) <= kMaxOffset && ld32u(base + (c - s0)) == v;
       }
       __builtin_amdgcn_wave_barrier();
       if (active)
@@ -228,8 +254,10 @@ __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ 
       const int first = __ffsll((unsigned long long)mask) - 1;
       uint32_t mp = pos + (uint32_t)first * stride;
       uint32_t mc = wave_bcast(cand, first) - 1;
-      uint32_t len = wave_match_extend(in, mp, mc, 4, match_end);
-      while (mp > anchor && mc > 0 && (mp - mc) <= kMaxOffset && in[mp - 1] == in[mc - 1]) {
+      uint32_t len = wave_match_extend(base, mp - s0, mc - s0, 4, match_end - s0);
+      // mp - 1 >= anchor >= s0 is staged; mc - 1 may lie before the segment
+      while (mp > anchor && mc > 0 && (mp - mc) <= kMaxOffset &&
+             base[mp - 1 - s0] == (mc - 1 >= s0 ? base[mc - 1 - s0] : in[mc - 1])) {
         --mp;
         --mc;
         ++len;
@@ -239,14 +267,14 @@ __global__ __launch_bounds__(64) void lz4_segment_k(const uint8_t* __restrict__ 
         tok0 = len - 4 >= 15 ? 15 : len - 4;
         op = emit_match_part(out, op, len, mp - mc);
       } else {
-        op = emit(out, op, in, anchor, mp - anchor, true, len, mp - mc);
+        op = emit(out, op, base + (anchor - s0), 0, mp - anchor, true, len, mp - mc);
       }
       ++nseq;
       pos = mp + len;
       anchor = pos;
       if (lane == 0 && pos - 2 <= last_start) {
         const uint32_t q = pos - 2;
-        table[lz4_hash(ld32u(in + q), HASH_LOG)] = q + 1;
+        table[lz4_hash(ld32u(base + (q - s0)), HASH_LOG)] = q + 1;
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -274,83 +302,119 @@ __device__ __forceinline__ void block_copy(uint8_t* dst, const uint8_t* src, uin
     dst[t0 + t] = src[t0 + t];
 }
 
-// Where the stitched block puts one segment's pieces: its literal run (input
-// bytes [lit_src, lit_src + lit_len), carried from earlier segments without
-// a match) at lit_dst, its sequences (scratch) at body_dst. Entry nseg of a
-// chunk is the block's closing literal run (body_len 0).
+// Where the stitched block puts the bytes of one input segment. Part a: the
+// literals of the segment's first sequence that lie inside it, input
+// [a_src, a_src + a_len). Part b: the input bytes after its last match (all
+// of it when it holds no match), [b_src, b_src + b_len), which open the
+// literal run of the next segment with a match (or the block's closing run).
+// Body: the segment's sequences from scratch. Every input byte that is a
+// literal lies in exactly one part a or b, so the copies are disjoint.
 struct SegPlace
 {
-  uint32_t lit_src, lit_len, lit_dst, body_dst, body_len;
+  uint32_t a_src, a_len, a_dst, b_src, b_len, b_dst, body_dst, body_len;
 };
 
-// One thread per chunk: walks the chunk's segment records in order, writes
-// each literal run's token (with the first match's length nibble) and length
-// extension into the block, and records where the literal and sequence bytes
-// go; the closing literal-only sequence ends the block (LZ4 end rules).
+// One wavefront per chunk, 64 segments per step. A segment with matches
+// ("live") opens a literal run that starts where the previous live segment's
+// matches ended (its carry-in, found by ballot), so its token and length
+// bytes take a known number of bytes and an exclusive scan over the wave
+// gives every segment's output position. A second, backward pass gives each
+// segment's part b the output position of the run it belongs to: delta =
+// output position - input position is constant along one literal run.
 __global__ __launch_bounds__(64) void lz4_layout_k(uint32_t n, uint32_t seg_len, uint32_t nseg,
-                                                   uint64_t nchunks, const SegRec* __restrict__ recs,
-                                                   SegPlace* __restrict__ places, uint8_t* __restrict__ dst,
-                                                   uint64_t dst_stride, uint32_t* __restrict__ sizes)
+                                                   const SegRec* __restrict__ recs_all,
+                                                   SegPlace* __restrict__ places_all,
+                                                   uint8_t* __restrict__ dst, uint64_t dst_stride,
+                                                   uint32_t* __restrict__ sizes)
 {
-  const uint64_t chunk = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-  if (chunk >= nchunks)
-    return;
+  const uint64_t chunk = blockIdx.x;
+  const int lane = lane_id();
   uint8_t* out = dst + chunk * dst_stride;
-  SegPlace* pl = places + chunk * (nseg + 1);
-  uint32_t op = 0, carry = 0; // carry: input position where pending literals start
-  for (uint32_t seg = 0; seg < nseg; ++seg) {
-    const SegRec r = recs[chunk * nseg + seg];
-    if (r.nseq == 0) {
-      pl[seg] = SegPlace{ 0, 0, 0, 0, 0 };
-      continue;
-    }
+  const SegRec* recs = recs_all + chunk * nseg;
+  SegPlace* places = places_all + chunk * nseg;
+  const uint64_t below = (1ull << lane) - 1;
+  uint32_t op = 0, carry = 0; // uniform: output cursor, start of the pending literals
+  for (uint32_t w0 = 0; w0 < nseg; w0 += kWave) {
+    const uint32_t seg = w0 + (uint32_t)lane;
+    SegRec r{ 0, 0, 0, 0, 0 };
+    if (seg < nseg)
+      r = recs[seg];
+    const bool live = seg < nseg && r.nseq != 0;
     const uint32_t s0 = seg * seg_len;
-    const uint32_t lit = s0 + r.ll0 - carry;
-    out[op] = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | r.tok0);
-    op = lit >= 15 ? put_len_ext(out, op + 1, lit) : op + 1;
-    pl[seg] = SegPlace{ carry, lit, op, op + lit, r.body };
-    op += lit + r.body;
-    const uint32_t s1 = n - s0 < seg_len ? n : s0 + seg_len;
-    carry = s1 - r.tail;
+    const uint32_t s1 = seg < nseg ? (n - s0 < seg_len ? n : s0 + seg_len) : s0;
+    const uint32_t end = s1 - r.tail; // where this segment's pending literals start, if live
+    const uint64_t live_mask = wave_ballot(live);
+    const uint64_t lower = live_mask & below;
+    const int j = lower ? 63 - __clzll((long long)lower) : 0;
+    const uint32_t end_j = __shfl(end, j, kWave);
+    const uint32_t cin = lower ? end_j : carry;
+    const uint32_t lit = live ? s0 + r.ll0 - cin : 0;
+    const uint32_t hdr = live ? 1 + (lit >= 15 ? 1 + (lit - 15) / 255 : 0) : 0;
+    const uint32_t bytes = hdr + lit + (live ? r.body : 0);
+    const uint32_t incl = wave_incl_scan(bytes);
+    const uint32_t at = op + incl - bytes;
+    if (live) {
+      out[at] = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | r.tok0);
+      if (lit >= 15)
+        put_len_ext(out, at + 1, lit);
+    }
+    if (seg < nseg) {
+      const uint32_t delta = at + hdr - cin; // output - input along this run (mod 2^32)
+      places[seg] = live ? SegPlace{ s0, r.ll0, s0 + delta, end, r.tail, 0, at + hdr + lit, r.body }
+                         : SegPlace{ s0, 0, 0, s0, s1 - s0, 0, 0, 0 };
+    }
+    op += wave_bcast(incl, kWave - 1);
+    if (live_mask)
+      carry = wave_bcast(end, 63 - __clzll((long long)live_mask));
   }
   const uint32_t lit = n - carry;
-  out[op] = (uint8_t)((lit >= 15 ? 15 : lit) << 4);
-  op = lit >= 15 ? put_len_ext(out, op + 1, lit) : op + 1;
-  pl[nseg] = SegPlace{ carry, lit, op, op + lit, 0 };
-  sizes[chunk] = op + lit;
+  const uint32_t hdr = 1 + (lit >= 15 ? 1 + (lit - 15) / 255 : 0);
+  if (lane == 0) {
+    out[op] = (uint8_t)((lit >= 15 ? 15 : lit) << 4);
+    if (lit >= 15)
+      put_len_ext(out, op + 1, lit);
+    sizes[chunk] = op + hdr + lit;
+  }
+  uint32_t next_delta = op + hdr - carry; // the closing run's
+  // the places read back here were written by the same lanes above
+  for (uint32_t w0 = (nseg - 1) / kWave * kWave;; w0 -= kWave) {
+    const uint32_t seg = w0 + (uint32_t)lane;
+    SegPlace pl{};
+    if (seg < nseg)
+      pl = places[seg];
+    const bool is_live = seg < nseg && recs[seg].nseq != 0;
+    const uint32_t delta = pl.a_dst - pl.a_src;
+    const uint64_t live_mask = wave_ballot(is_live);
+    const uint64_t upper = lane == kWave - 1 ? 0 : live_mask & (~0ull << (lane + 1));
+    const int k = upper ? __ffsll((unsigned long long)upper) - 1 : 0;
+    const uint32_t delta_k = __shfl(delta, k, kWave);
+    if (seg < nseg)
+      places[seg].b_dst = pl.b_src + (upper ? delta_k : next_delta);
+    if (live_mask)
+      next_delta = wave_bcast(delta, __ffsll((unsigned long long)live_mask) - 1);
+    if (w0 == 0)
+      break;
+  }
 }
 
-constexpr uint32_t kStitchPiece = 16 * 1024;
-
-// Grid-stride over (chunk, segment, part, piece): part 0 copies a piece of
-// the segment's literal run from the input, part 1 a piece of its sequences
-// from scratch, each to its place in the block.
+// One workgroup per (chunk, segment): the segment's literal parts a and b
+// from the input and its sequences from scratch, each to its place.
 __global__ __launch_bounds__(256) void lz4_copy_k(const uint8_t* __restrict__ src, uint64_t src_stride,
-                                                  uint32_t nseg, uint64_t nchunks, uint32_t lit_pieces,
-                                                  uint32_t body_pieces, const uint8_t* __restrict__ tmp,
+                                                  uint32_t nseg, uint64_t total, const uint8_t* __restrict__ tmp,
                                                   uint64_t seg_bound, const SegPlace* __restrict__ places,
                                                   uint8_t* __restrict__ dst, uint64_t dst_stride)
 {
-  const uint64_t per_seg = (uint64_t)lit_pieces + body_pieces;
-  const uint64_t total = nchunks * (nseg + 1) * per_seg;
-  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
-    const uint64_t cs = w / per_seg; // chunk * (nseg + 1) + seg
-    const uint32_t k = (uint32_t)(w - cs * per_seg);
-    const uint64_t chunk = cs / (nseg + 1);
-    const uint32_t seg = (uint32_t)(cs - chunk * (nseg + 1));
-    const SegPlace pl = places[cs];
+  for (uint64_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const uint64_t chunk = item / nseg;
+    const SegPlace pl = places[item];
+    const uint8_t* in = src + chunk * src_stride;
     uint8_t* out = dst + chunk * dst_stride;
-    if (k < lit_pieces) {
-      const uint32_t off = k * kStitchPiece;
-      if (off < pl.lit_len)
-        block_copy(out + pl.lit_dst + off, src + chunk * src_stride + pl.lit_src + off,
-                   pl.lit_len - off < kStitchPiece ? pl.lit_len - off : kStitchPiece);
-    } else {
-      const uint32_t off = (k - lit_pieces) * kStitchPiece;
-      if (off < pl.body_len)
-        block_copy(out + pl.body_dst + off, tmp + (chunk * nseg + seg) * seg_bound + off,
-                   pl.body_len - off < kStitchPiece ? pl.body_len - off : kStitchPiece);
-    }
+    if (pl.a_len)
+      block_copy(out + pl.a_dst, in + pl.a_src, pl.a_len);
+    if (pl.b_len)
+      block_copy(out + pl.b_dst, in + pl.b_src, pl.b_len);
+    if (pl.body_len)
+      block_copy(out + pl.body_dst, tmp + item * seg_bound, pl.body_len);
   }
 }
 
@@ -369,7 +433,7 @@ size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes)
   if (!seg_bytes || seg_bytes >= chunk_bytes)
     return 0;
   const uint64_t nseg = (chunk_bytes + seg_bytes - 1) / seg_bytes;
-  return n * nseg * (lz4_segment_bound(seg_bytes) + sizeof(SegRec)) + n * (nseg + 1) * sizeof(SegPlace) + 256;
+  return n * nseg * (lz4_segment_bound(seg_bytes) + sizeof(SegRec)) + n * nseg * sizeof(SegPlace) + 256;
 }
 
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
@@ -401,27 +465,34 @@ hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_by
   const uint64_t bound = lz4_segment_bound(seg_bytes);
   uint8_t* tmp = (uint8_t*)scratch;
   SegRec* recs = (SegRec*)(tmp + total * bound);
-#define LZ4_SEG(H)                                                                            \
-  hipLaunchKernelGGL(lz4_segment_k<H>, dim3((uint32_t)total), dim3(64), 0, stream, src,         \
+  // segments that fit the staging buffer are parsed out of LDS
+#define LZ4_SEG(H, S)                                                                         \
+  hipLaunchKernelGGL((lz4_segment_k<H, S>), dim3((uint32_t)total), dim3(64), 0, stream, src,    \
                      src_stride, (uint32_t)chunk_bytes, seg_bytes, nseg, total, tmp, bound, recs)
-  switch (hash_log) {
-    case 10: LZ4_SEG(10); break;
-    case 11: LZ4_SEG(11); break;
-    case 13: LZ4_SEG(13); break;
-    default: LZ4_SEG(12); break;
+#define LZ4_SEG_H(S)                                                                          \
+  switch (hash_log) {                                                                         \
+    case 10: LZ4_SEG(10, S); break;                                                           \
+    case 11: LZ4_SEG(11, S); break;                                                           \
+    case 13: LZ4_SEG(13, S); break;                                                           \
+    default: LZ4_SEG(12, S); break;                                                           \
   }
+  if (seg_bytes <= kLz4StageSmall) {
+    LZ4_SEG_H(kLz4StageSmall)
+  } else if (seg_bytes <= kLz4StageMax) {
+    LZ4_SEG_H(kLz4StageMax)
+  } else {
+    LZ4_SEG_H(0)
+  }
+#undef LZ4_SEG_H
 #undef LZ4_SEG
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   SegPlace* places = (SegPlace*)(((uintptr_t)(recs + total) + 15) & ~(uintptr_t)15);
-  hipLaunchKernelGGL(lz4_layout_k, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, stream, (uint32_t)chunk_bytes,
-                     seg_bytes, nseg, n, recs, places, dst, dst_stride, sizes);
-  const uint32_t lit_pieces = (uint32_t)((chunk_bytes + kStitchPiece - 1) / kStitchPiece);
-  const uint32_t body_pieces = (uint32_t)((bound + kStitchPiece - 1) / kStitchPiece);
-  const uint64_t items = n * (nseg + 1) * ((uint64_t)lit_pieces + body_pieces);
-  hipLaunchKernelGGL(lz4_copy_k, dim3((uint32_t)std::min<uint64_t>(items, 1u << 16)), dim3(256), 0, stream,
-                     src, src_stride, nseg, n, lit_pieces, body_pieces, tmp, bound, places, dst, dst_stride);
+  hipLaunchKernelGGL(lz4_layout_k, dim3((uint32_t)n), dim3(64), 0, stream, (uint32_t)chunk_bytes, seg_bytes, nseg,
+                     recs, places, dst, dst_stride, sizes);
+  hipLaunchKernelGGL(lz4_copy_k, dim3((uint32_t)std::min<uint64_t>(total, 1u << 20)), dim3(256), 0, stream, src,
+                     src_stride, nseg, total, tmp, bound, places, dst, dst_stride);
   return hipGetLastError();
 }
 

@@ -820,7 +820,16 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     s->seg_stride = s->chunk_slot;
     s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : 12;
     const uint64_t per_launch = chunks_per_launch(s);
-    s->lz4_seg = tune_lz4_segment(s->chunk_bytes, per_launch, s->lz4_hash_log);
+    if (s->opts.lz4_segment_bytes) {
+      const uint32_t sb = s->opts.lz4_segment_bytes;
+      if (sb < s->chunk_bytes && (sb < 1024 || (sb & (sb - 1)))) {
+        set_error("create: lz4_segment_bytes must be a power of two >= 1024, or >= the chunk bytes");
+        return 1;
+      }
+      s->lz4_seg = sb < s->chunk_bytes ? sb : 0;
+    } else {
+      s->lz4_seg = tune_lz4_segment(s->chunk_bytes, per_launch, s->lz4_hash_log);
+    }
     if (s->lz4_seg && dalloc(s, (void**)&s->d_lz4_scratch,
                              lz4_scratch_bytes(s->chunk_bytes, per_launch, s->lz4_seg)))
       return 1;
@@ -1083,8 +1092,9 @@ size_t tile_stream_gpu_epoch_latencies(const struct tile_stream_gpu* s, double* 
   return n;
 }
 
-int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* config,
-                                    struct tile_stream_memory_info* info)
+namespace {
+int estimate_core(const struct tile_stream_configuration* config, struct tile_stream_memory_info* info,
+                  uint32_t lz4_segment_override)
 {
   if (!info || !config)
     return 1;
@@ -1131,7 +1141,9 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
     if (p.max_cursor_elements && E)
       epochs = std::min<uint64_t>(epochs, (p.max_cursor_elements + E - 1) / E);
     const uint64_t per_launch = std::max<uint64_t>(epochs, 1) * p.total_chunks;
-    const uint32_t seg = tune_lz4_segment(cb, per_launch, 12);
+    const uint32_t seg = lz4_segment_override
+                           ? (lz4_segment_override < cb ? lz4_segment_override : 0)
+                           : tune_lz4_segment(cb, per_launch, 12);
     if (seg)
       info->codec_bytes += lz4_scratch_bytes(cb, per_launch, seg);
   }
@@ -1150,11 +1162,19 @@ int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* conf
   return 0;
 }
 
+} // namespace
+
+int tile_stream_gpu_memory_estimate(const struct tile_stream_configuration* config,
+                                    struct tile_stream_memory_info* info)
+{
+  return estimate_core(config, info, 0);
+}
+
 int tile_stream_gpu_memory_estimate_ex(const struct tile_stream_configuration* config,
                                        const struct tile_stream_gpu_options* opts,
                                        struct tile_stream_memory_info* info)
 {
-  if (tile_stream_gpu_memory_estimate(config, info))
+  if (estimate_core(config, info, opts ? opts->lz4_segment_bytes : 0))
     return 1;
   // the batched pipeline (which aligned streams do not use) adds, per host
   // slot, one epoch's worst-case bytes of every level in pinned memory and

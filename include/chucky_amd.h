@@ -39,6 +39,9 @@ struct tile_stream_gpu_options
                                       1 = gather kernel writing pinned host memory */
   uint32_t batched_launch;         /* batched pipeline per-epoch work: 0 = autotuned,
                                       1 = captured graphs, 2 = direct kernel launches */
+  uint32_t lz4_segment_bytes;      /* LZ4 input parsed by one wavefront: a power of two
+                                      >= 1024 below the chunk size, or >= the chunk size for
+                                      one wavefront per chunk; 0 = autotuned */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);
@@ -159,14 +162,17 @@ size_t chucky_gpu_max_output_size(enum compression_codec codec, size_t chunk_byt
 
 /* Compress n chunks: chunk i is chunk_bytes at d_src + i*src_stride, output at
  * d_dst + i*dst_stride (dst_stride >= chucky_gpu_max_output_size), sizes into
- * d_sizes[i] (uint64). 0 on success. */
+ * d_sizes[i] (uint64). The compressors read whole dwords and 16-byte pieces,
+ * so d_src must stay readable for 64 bytes past the last chunk's end (the
+ * bytes there do not change the output). 0 on success. */
 int chucky_gpu_compress(enum compression_codec codec, int level, const void* d_src,
                         size_t src_stride, size_t chunk_bytes, uint64_t n, void* d_dst,
                         size_t dst_stride, uint64_t* d_sizes);
 
 /* No counterpart in the reference: chucky_gpu_compress for callers that
  * cannot allocate (code under HIP graph capture, tight serving loops). The
- * same inputs, outputs and compressed bytes, with every other buffer in one
+ * same inputs (with the same 64 bytes of read slack), outputs and compressed
+ * bytes, with every other buffer in one
  * file-scope device array of the library (about 25 MiB a device, held until
  * the process ends): no hipMalloc, hipFree or stream creation. Asynchronous
  * on `stream` (a hipStream_t; NULL = the null stream): synchronize it before

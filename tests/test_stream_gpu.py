@@ -234,6 +234,33 @@ def test_lz4_multi_wavefront_chunks(name, pipeline):
     _compare(cfg, LZ4_SEGMENT_DATA[name](), pieces=2, pipeline=pipeline)
 
 
+LZ4_SEGMENT_SHAPES = {
+    # 128 KiB chunks: up to 128 segments a chunk (two wavefront windows of the layout pass)
+    "pow2": ([Dim(64, 32, 2), Dim(64, 32, 2), Dim(64, 64, 1)], "u16", (64, 64, 64)),
+    # 46080-byte chunks: a partial last segment
+    "ragged": ([Dim(48, 24, 2), Dim(48, 24, 2), Dim(40, 40, 1)], "u16", (48, 48, 40)),
+    # 1029-byte chunks: with 1 KiB segments the last one is too short to parse
+    "tiny_tail": ([Dim(6, 3, 2), Dim(14, 7, 2), Dim(98, 49, 2)], "u8", (6, 14, 98)),
+}
+
+
+@pytest.mark.parametrize("shape", sorted(LZ4_SEGMENT_SHAPES))
+@pytest.mark.parametrize("seg,hash_log", [(1024, 10), (4096, 11), (8192, 12), (16384, 12), (32768, 13),
+                                          (1 << 20, 12)])
+@pytest.mark.parametrize("data", ["xor", "thirds"])
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_lz4_segment_sizes_and_hash_logs(shape, seg, hash_log, data, pipeline):
+    # every LZ4 segment size (LDS-staged up to 16 KiB, parsed from HBM above,
+    # one wavefront a chunk when the segment covers the chunk) and table size
+    dims, dtype, vol = LZ4_SEGMENT_SHAPES[shape]
+    v = (xor_volume if data == "xor" else thirds_volume)(vol)
+    if dtype == "u8":
+        v = (v & 0xFF).astype(np.uint8)
+    cfg = ca.Config(dims, dtype=dtype, codec="lz4")
+    _compare(cfg, v.reshape(-1), pieces=2, pipeline=pipeline, lz4_segment_bytes=seg,
+             lz4_hash_log=hash_log)
+
+
 ZSTD_MODE_DATA = {
     "xor": lambda: xor_volume((32, 128, 128)),
     "sensor_like": lambda: np.clip(np.random.default_rng(4).poisson(100, 32 * 128 * 128), 0, 4095)
