@@ -195,6 +195,8 @@ _sig("chucky_gpu_transpose", C.c_int, _P, _P, C.c_uint64, C.c_uint8, C.c_uint64,
      C.POINTER(C.c_uint64), C.POINTER(C.c_int64))
 _sig("chucky_crc32c", C.c_uint32, _P, C.c_size_t)
 _sig("dims_n_append", C.c_uint8, C.POINTER(Dimension), C.c_uint8)
+_sig("chucky_tune_codec", C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64,
+     C.POINTER(C.c_uint32))
 _sig("chucky_device_alloc", _P, C.c_size_t)
 _sig("chucky_device_free", None, _P)
 _sig("chucky_memcpy_to_device", C.c_int, _P, _P, C.c_size_t)
@@ -258,7 +260,7 @@ class DeviceBuffer:
         self.free()
 
 from .api import (Config, GpuStream, GpuMultiArray, MemSink, DiscardSink, last_error,  # noqa: E402
-                  crc32c, memory_estimate)
+                  crc32c, memory_estimate, tuned_codec)
 
 __all__ = ["lib", "PIPELINE_PLAIN", "PIPELINE_BATCHED", "DeviceBuffer", "Config", "GpuStream", "GpuMultiArray", "MemSink", "DiscardSink", "last_error", "crc32c",
-           "memory_estimate", "DTYPES", "CODEC_ID", "REDUCE_ID"]
+           "memory_estimate", "tuned_codec", "DTYPES", "CODEC_ID", "REDUCE_ID"]

@@ -161,6 +161,16 @@ class DiscardSink:
         self.close()
 
 
+def tuned_codec(codec: str, bpe: int, chunk_bytes: int, chunks_per_launch: int,
+                multiscale: bool = False) -> Tuple[int, int]:
+    """The autotuner's launch shape: zstd (block bytes, chunk tables 0/1), lz4
+    (segment bytes or 0 for whole chunks, hash log)."""
+    out = (C.c_uint32 * 2)()
+    if lib.chucky_tune_codec(CODEC_ID[codec], int(multiscale), bpe, chunk_bytes, chunks_per_launch, out):
+        raise ValueError("chucky_tune_codec: invalid arguments")
+    return out[0], out[1]
+
+
 def memory_estimate(cfg: Config, pipeline: Optional[int] = None) -> MemoryInfo:
     """tile_stream_gpu_memory_estimate; with a pipeline, the estimate of what
     that pipeline allocates (tile_stream_gpu_memory_estimate_ex)."""

@@ -1,19 +1,124 @@
 // Launch-shape choices for the codec kernels.
 //
-// The compressors give one wavefront to each block of a chunk, and a
-// persistent grid of `compressor_slots()` wavefronts pulls blocks from a
-// device queue. A launch finishes after ceil(blocks / slots) rounds of the
-// slowest block, so when a launch holds few chunks (a per-epoch launch of a
-// small epoch, a short bounded stream) smaller blocks put more wavefronts to
-// work; when it holds many, larger blocks cost less per byte (one Huffman
-// table and one header per block) and compress better.
+// The compressors give one wavefront to each block (zstd) or segment (LZ4)
+// of a chunk. A launch finishes after its slowest rounds of blocks, so a
+// launch that holds few chunks (a per-epoch launch of a small epoch, a short
+// bounded stream) wants small blocks to put every CU to work, and one that
+// holds many can afford larger blocks, which cost less per byte and compress
+// better. LZ4 speed is also set by how many wavefronts fit a CU: each holds
+// its hash table and its staged segment in LDS.
+//
+// The choices come from recorded sweeps (chucky_amd/tune.py, run on MI355X:
+// `python -m chucky_amd.tune --codec lz4|zstd --out F`, then `--table F`):
+// every dtype size x chunk size x chunks per launch below was streamed once
+// per candidate over two synthetic data kinds (xor+noise volumes, Poisson
+// sensor counts), and the row keeps the candidate with the least time per
+// input byte through the compressor plus its output's time over the 55 GB/s
+// device-to-host link. Multiscale streams have rows of their own (every
+// level in one launch; the coarse levels' smooth chunks parse into dense
+// sequences and set the launch's tail). A stream takes the row nearest its
+// own case (multiscale or not, then dtype size, then chunk bytes and chunks
+// per launch on a log scale).
+#include "chucky_amd.h"
+#include "kernels.hpp"
 #include "stream_gpu.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 namespace chucky {
+
+namespace {
+
+struct Lz4Row
+{
+  uint32_t lod, bpe, chunk_bytes, per_launch, seg_bytes, hash_log;
+};
+
+// multiscale (1) or single-level (0) stream, bpe, chunk bytes, chunks per
+// launch -> segment bytes (0 = whole chunk), hash log; the comment is the
+// sweep's mean rate and ratio for the choice
+const Lz4Row kLz4Table[] = {
+  { 0, 1, 32768, 16, 1024, 9 }, // 28.9 GB/s, 0.996
+  { 0, 1, 32768, 64, 1024, 9 }, // 101.5 GB/s, 0.996
+  { 0, 1, 32768, 256, 1024, 9 }, // 261.1 GB/s, 0.996
+  { 0, 1, 262144, 16, 1024, 9 }, // 137.9 GB/s, 0.996
+  { 0, 1, 262144, 64, 2048, 9 }, // 382.5 GB/s, 0.996
+  { 0, 1, 262144, 256, 4096, 9 }, // 663.7 GB/s, 0.996
+  { 0, 2, 65536, 16, 1024, 9 }, // 35.5 GB/s, 1.012
+  { 0, 2, 65536, 64, 1024, 9 }, // 107.2 GB/s, 1.012
+  { 0, 2, 65536, 256, 1024, 9 }, // 200.7 GB/s, 1.012
+  { 0, 2, 524288, 16, 2048, 9 }, // 137.0 GB/s, 1.013
+  { 0, 2, 524288, 64, 2048, 9 }, // 269.2 GB/s, 1.013
+  { 0, 2, 524288, 256, 2048, 9 }, // 328.6 GB/s, 1.013
+  { 0, 4, 131072, 16, 1024, 9 }, // 42.1 GB/s, 1.072
+  { 0, 4, 131072, 64, 1024, 9 }, // 92.9 GB/s, 1.072
+  { 0, 4, 131072, 256, 1024, 9 }, // 126.6 GB/s, 1.073
+  { 0, 4, 1048576, 16, 1024, 9 }, // 119.5 GB/s, 1.061
+  { 0, 4, 1048576, 64, 1024, 9 }, // 167.8 GB/s, 1.061
+  { 0, 4, 1048576, 256, 1024, 9 }, // 173.4 GB/s, 1.061
+  { 1, 2, 1024, 1365, 0, 9 }, // 22.1 GB/s, 1.168
+  { 1, 2, 1024, 5460, 0, 9 }, // 72.3 GB/s, 1.168
+  { 1, 2, 65536, 85, 1024, 9 }, // 77.1 GB/s, 1.180
+  { 1, 2, 65536, 340, 1024, 9 }, // 162.5 GB/s, 1.180
+  { 1, 2, 524288, 21, 1024, 9 }, // 110.8 GB/s, 1.167
+  { 1, 2, 524288, 84, 1024, 9 }, // 208.3 GB/s, 1.167
+};
+
+struct ZstdRow
+{
+  uint32_t lod, bpe, chunk_bytes, per_launch, block, chunk_tables;
+};
+
+// multiscale or not, bpe, chunk bytes, chunks per launch -> block bytes, one
+// Huffman table per chunk
+const ZstdRow kZstdTable[] = {
+  { 0, 1, 32768, 16, 8192, 0 }, // 6.3 GB/s, 1.470
+  { 0, 1, 32768, 64, 8192, 0 }, // 23.3 GB/s, 1.470
+  { 0, 1, 32768, 256, 8192, 0 }, // 73.3 GB/s, 1.470
+  { 0, 1, 262144, 16, 8192, 0 }, // 41.8 GB/s, 1.333
+  { 0, 1, 262144, 64, 16384, 0 }, // 113.2 GB/s, 1.339
+  { 0, 1, 262144, 256, 32768, 0 }, // 248.5 GB/s, 1.342
+  { 0, 2, 65536, 16, 8192, 0 }, // 11.0 GB/s, 1.804
+  { 0, 2, 65536, 64, 8192, 0 }, // 38.4 GB/s, 1.803
+  { 0, 2, 65536, 256, 8192, 0 }, // 99.7 GB/s, 1.803
+  { 0, 2, 524288, 16, 8192, 0 }, // 64.7 GB/s, 1.665
+  { 0, 2, 524288, 64, 16384, 0 }, // 160.5 GB/s, 1.675
+  { 0, 2, 524288, 256, 65536, 0 }, // 274.7 GB/s, 1.682
+  { 0, 4, 131072, 16, 8192, 0 }, // 8.2 GB/s, 2.329
+  { 0, 4, 131072, 64, 8192, 0 }, // 29.4 GB/s, 2.309
+  { 0, 4, 131072, 256, 8192, 1 }, // 55.6 GB/s, 2.336
+  { 0, 4, 1048576, 16, 8192, 0 }, // 56.2 GB/s, 2.185
+  { 0, 4, 1048576, 64, 8192, 0 }, // 67.9 GB/s, 2.184
+  { 0, 4, 1048576, 256, 8192, 1 }, // 99.6 GB/s, 2.211
+  { 1, 2, 1024, 1365, 8192, 0 }, // 11.8 GB/s, 2.116
+  { 1, 2, 1024, 5460, 16384, 0 }, // 20.9 GB/s, 2.116
+  { 1, 2, 65536, 85, 8192, 0 }, // 11.8 GB/s, 2.131
+  { 1, 2, 65536, 340, 8192, 0 }, // 38.2 GB/s, 2.131
+  { 1, 2, 524288, 21, 8192, 0 }, // 25.5 GB/s, 1.993
+  { 1, 2, 524288, 84, 8192, 0 }, // 65.6 GB/s, 1.993
+};
+
+template<typename Row, size_t N>
+const Row& nearest(const Row (&table)[N], bool lod, uint32_t bpe, uint64_t chunk_bytes, uint64_t per_launch)
+{
+  const auto dist = [&](const Row& r) {
+    const double dm = (r.lod != 0) != lod ? 16.0 : 0.0;
+    const double db = std::fabs(std::log2((double)std::max<uint32_t>(bpe, 1) / r.bpe));
+    const double dc = std::fabs(std::log2((double)std::max<uint64_t>(chunk_bytes, 1) / r.chunk_bytes));
+    const double dl = std::fabs(std::log2((double)std::max<uint64_t>(per_launch, 1) / r.per_launch));
+    return dm + 4 * db + dc + dl;
+  };
+  size_t best = 0;
+  for (size_t i = 1; i < N; ++i)
+    if (dist(table[i]) < dist(table[best]))
+      best = i;
+  return table[best];
+}
+
+} // namespace
 
 uint32_t compressor_slots()
 {
@@ -24,62 +129,29 @@ uint32_t compressor_slots()
   return (uint32_t)cus * 15;
 }
 
-uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch)
+uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod)
 {
-  const uint64_t slots = compressor_slots();
-  // Largest block up to 32 KiB that still fills every slot once, else the
-  // smallest block. Measured on 12-bit sensor tiles: 32 KiB blocks compress
-  // faster than 128 KiB ones (more wavefronts in flight, shorter serial
-  // Huffman steps per byte) and slightly better (more local tables).
-  // 8 KiB is the floor: 4 KiB blocks (allowed when asked for) measured slower
-  // on latency-bound launches too (twice the per-block Huffman work) and
-  // compress 1% worse.
-  constexpr uint32_t kPreferred = 32 * 1024, kFloor = 8 * 1024;
-  for (uint32_t b = kPreferred; b > kFloor; b >>= 1) {
-    const uint64_t blocks = chunks_per_launch * ((chunk_bytes + b - 1) / b);
-    if (blocks >= slots)
-      return b;
-  }
-  return kFloor;
+  return nearest(kZstdTable, lod, bpe, chunk_bytes, chunks_per_launch).block;
 }
 
-uint64_t lz4_wave_slots(int hash_log)
+bool tune_zstd_chunk_tables(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod)
+{
+  return nearest(kZstdTable, lod, bpe, chunk_bytes, chunks_per_launch).chunk_tables != 0;
+}
+
+Lz4Choice tune_lz4(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod)
+{
+  const Lz4Row& r = nearest(kLz4Table, lod, bpe, chunk_bytes, chunks_per_launch);
+  return Lz4Choice{ r.seg_bytes && r.seg_bytes < chunk_bytes ? r.seg_bytes : 0u, (int)r.hash_log };
+}
+
+uint64_t lz4_wave_slots(int hash_log, uint32_t seg_bytes)
 {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // one LZ4 wavefront holds a (4 << hash_log)-byte LDS table
-  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / (4ull << hash_log));
+  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / lz4_lds_bytes(hash_log, seg_bytes));
   return (uint64_t)cus * std::min<uint64_t>(per_cu, 16);
-}
-
-uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log)
-{
-  const uint64_t slots = lz4_wave_slots(hash_log);
-  if (chunks_per_launch >= slots)
-    return 0;
-  // 8 KiB measured fastest end to end on one-epoch launches of 512 KiB
-  // chunks; 4 KiB segments parse faster but lose more to the stitch pass
-  constexpr uint32_t kMinSeg = 8 * 1024;
-  uint32_t seg = 0;
-  for (uint64_t b = 1ull << 30; b >= kMinSeg; b >>= 1) {
-    if (b >= chunk_bytes)
-      continue;
-    seg = (uint32_t)b;
-    if (chunks_per_launch * ((chunk_bytes + b - 1) / b) >= slots)
-      break;
-  }
-  return seg;
-}
-
-// Per-block Huffman tables adapt to local statistics; since the code
-// construction runs wave-parallel (register queues, pointer-jumping depths)
-// they beat one table per chunk in both speed and ratio on every measured
-// workload, so the per-chunk tables are opt-in.
-bool tune_zstd_chunk_tables(uint32_t block)
-{
-  (void)block;
-  return false;
 }
 
 uint32_t tune_host_gather_workgroups()
@@ -92,3 +164,23 @@ uint32_t tune_host_gather_workgroups()
 }
 
 } // namespace chucky
+
+extern "C" int chucky_tune_codec(enum compression_codec codec, int multiscale, uint32_t bpe,
+                                 uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t out[2])
+{
+  if (!out || bpe == 0 || chunk_bytes == 0 || chunks_per_launch == 0)
+    return 1;
+  const bool lod = multiscale != 0;
+  if (codec == CODEC_ZSTD) {
+    out[0] = chucky::tune_zstd_block(chunk_bytes, chunks_per_launch, bpe, lod);
+    out[1] = chucky::tune_zstd_chunk_tables(chunk_bytes, chunks_per_launch, bpe, lod) ? 1u : 0u;
+    return 0;
+  }
+  if (codec == CODEC_LZ4_NON_STANDARD) {
+    const chucky::Lz4Choice c = chucky::tune_lz4(chunk_bytes, chunks_per_launch, bpe, lod);
+    out[0] = c.seg_bytes;
+    out[1] = (uint32_t)c.hash_log;
+    return 0;
+  }
+  return 1;
+}

@@ -104,6 +104,9 @@ uint64_t zstd_segment_stride(uint32_t block);
 // bytes) one wavefront per segment of seg_bytes plus a per-chunk stitch,
 // using `scratch` of lz4_scratch_bytes(chunk_bytes, n, seg_bytes).
 size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes);
+// LDS bytes of one LZ4 wavefront: the hash table, plus the staged input of a
+// segment when segments of seg_bytes (0 = whole chunks) are parsed from LDS.
+uint32_t lz4_lds_bytes(int hash_log, uint32_t seg_bytes);
 hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                       uint8_t* dst, uint64_t dst_stride, uint32_t* sizes, int hash_log,
                       uint32_t seg_bytes, void* scratch, hipStream_t stream);

@@ -26,9 +26,16 @@ namespace {
 constexpr uint32_t kMaxOffset = 65535;
 constexpr uint32_t kMfLimit = 12;
 constexpr uint32_t kLastLiterals = 5;
-// LDS staging capacities of the segment parser (bytes of input per wavefront)
-constexpr int kLz4StageSmall = 8 * 1024;
+// LDS staging capacities of the segment parser (bytes of input per
+// wavefront): a segment is staged in the smallest that holds it, so small
+// segments keep the wavefront's LDS small and more wavefronts fit a CU.
 constexpr int kLz4StageMax = 16 * 1024;
+
+constexpr int lz4_stage_for(uint32_t seg_bytes)
+{
+  return seg_bytes <= 2048 ? 2048 : seg_bytes <= 4096 ? 4096 : seg_bytes <= 8192 ? 8192
+                                                     : seg_bytes <= (uint32_t)kLz4StageMax ? kLz4StageMax : 0;
+}
 
 __device__ __forceinline__ uint32_t lz4_hash(uint32_t v, int hash_log)
 {
@@ -426,6 +433,12 @@ size_t lz4_max_output_size(size_t chunk_bytes)
   return chunk_bytes + chunk_bytes / 255 + 16;
 }
 
+uint32_t lz4_lds_bytes(int hash_log, uint32_t seg_bytes)
+{
+  const int stage = seg_bytes == 0 ? 0 : lz4_stage_for(seg_bytes);
+  return (4u << hash_log) + (stage ? (uint32_t)stage + 16 : 0u);
+}
+
 uint64_t lz4_segment_bound(uint32_t seg_bytes) { return (lz4_max_output_size(seg_bytes) + 15) & ~15ull; }
 
 size_t lz4_scratch_bytes(uint64_t chunk_bytes, uint64_t n, uint32_t seg_bytes)
@@ -450,6 +463,7 @@ hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_by
   hipLaunchKernelGGL(lz4_chunk_k<H>, dim3((uint32_t)n), dim3(64), 0, stream, src, src_stride,   \
                      (uint32_t)chunk_bytes, n, dst, dst_stride, sizes)
     switch (hash_log) {
+      case 9: LZ4_CHUNK(9); break;
       case 10: LZ4_CHUNK(10); break;
       case 11: LZ4_CHUNK(11); break;
       case 13: LZ4_CHUNK(13); break;
@@ -471,17 +485,18 @@ hipError_t launch_lz4(const uint8_t* src, uint64_t src_stride, uint64_t chunk_by
                      src_stride, (uint32_t)chunk_bytes, seg_bytes, nseg, total, tmp, bound, recs)
 #define LZ4_SEG_H(S)                                                                          \
   switch (hash_log) {                                                                         \
+    case 9: LZ4_SEG(9, S); break;                                                             \
     case 10: LZ4_SEG(10, S); break;                                                           \
     case 11: LZ4_SEG(11, S); break;                                                           \
     case 13: LZ4_SEG(13, S); break;                                                           \
     default: LZ4_SEG(12, S); break;                                                           \
   }
-  if (seg_bytes <= kLz4StageSmall) {
-    LZ4_SEG_H(kLz4StageSmall)
-  } else if (seg_bytes <= kLz4StageMax) {
-    LZ4_SEG_H(kLz4StageMax)
-  } else {
-    LZ4_SEG_H(0)
+  switch (lz4_stage_for(seg_bytes)) {
+    case 2048: LZ4_SEG_H(2048) break;
+    case 4096: LZ4_SEG_H(4096) break;
+    case 8192: LZ4_SEG_H(8192) break;
+    case kLz4StageMax: LZ4_SEG_H(kLz4StageMax) break;
+    default: LZ4_SEG_H(0) break;
   }
 #undef LZ4_SEG_H
 #undef LZ4_SEG

@@ -502,7 +502,7 @@ int batched_init(tile_stream_gpu* s)
     const uint64_t per_chunk = s->lz4_seg ? (s->chunk_bytes + s->lz4_seg - 1) / s->lz4_seg : 1;
     const uint64_t waves = std::max<uint64_t>((uint64_t)p.total_chunks * per_chunk, 1);
     b->NZ = (uint32_t)std::min<uint64_t>(std::min<uint32_t>(K, kMaxZip),
-                                         std::max<uint64_t>(1, lz4_wave_slots(s->lz4_hash_log) / waves));
+                                         std::max<uint64_t>(1, lz4_wave_slots(s->lz4_hash_log, s->lz4_seg) / waves));
     b->lz4_stride = align_up(lz4_scratch_bytes(s->chunk_bytes, p.total_chunks, s->lz4_seg), 256);
   }
   // HIP maps streams onto its hardware queues (GPU_MAX_HW_QUEUES, 4 by

@@ -818,8 +818,13 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     s->nseg = 1;
     s->chunk_slot = align_up(lz4_max_output_size(s->chunk_bytes), 256);
     s->seg_stride = s->chunk_slot;
-    s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : 12;
+    if (s->opts.lz4_hash_log && (s->opts.lz4_hash_log < 9 || s->opts.lz4_hash_log > 13)) {
+      set_error("create: lz4_hash_log must be in [9, 13]");
+      return 1;
+    }
     const uint64_t per_launch = chunks_per_launch(s);
+    const Lz4Choice tuned = tune_lz4(s->chunk_bytes, per_launch, (uint32_t)s->plan.bpe, s->plan.nlod > 1);
+    s->lz4_hash_log = s->opts.lz4_hash_log ? (int)s->opts.lz4_hash_log : tuned.hash_log;
     if (s->opts.lz4_segment_bytes) {
       const uint32_t sb = s->opts.lz4_segment_bytes;
       if (sb < s->chunk_bytes && (sb < 1024 || (sb & (sb - 1)))) {
@@ -828,7 +833,7 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
       }
       s->lz4_seg = sb < s->chunk_bytes ? sb : 0;
     } else {
-      s->lz4_seg = tune_lz4_segment(s->chunk_bytes, per_launch, s->lz4_hash_log);
+      s->lz4_seg = tuned.seg_bytes;
     }
     if (s->lz4_seg && dalloc(s, (void**)&s->d_lz4_scratch,
                              lz4_scratch_bytes(s->chunk_bytes, per_launch, s->lz4_seg)))
@@ -840,9 +845,12 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
       return 1;
     }
     s->zblock = s->opts.zstd_block_bytes ? s->opts.zstd_block_bytes
-                                         : tune_zstd_block(s->chunk_bytes, chunks_per_launch(s));
+                                         : tune_zstd_block(s->chunk_bytes, chunks_per_launch(s),
+                                                           (uint32_t)s->plan.bpe, s->plan.nlod > 1);
     const uint32_t ct = s->opts.zstd_chunk_tables;
-    const bool chunk_tables = ct == 1 || (ct == 0 && tune_zstd_chunk_tables(s->zblock));
+    const bool chunk_tables =
+      ct == 1 || (ct == 0 && tune_zstd_chunk_tables(s->chunk_bytes, chunks_per_launch(s), (uint32_t)s->plan.bpe,
+                                                    s->plan.nlod > 1));
     if (chunk_tables && dalloc(s, (void**)&s->d_ztables, zstd_table_bytes(chunks_per_launch(s))))
       return 1;
     s->nseg = zstd_segments(s->chunk_bytes, s->zblock);
@@ -1143,7 +1151,7 @@ int estimate_core(const struct tile_stream_configuration* config, struct tile_st
     const uint64_t per_launch = std::max<uint64_t>(epochs, 1) * p.total_chunks;
     const uint32_t seg = lz4_segment_override
                            ? (lz4_segment_override < cb ? lz4_segment_override : 0)
-                           : tune_lz4_segment(cb, per_launch, 12);
+                           : tune_lz4(cb, per_launch, (uint32_t)p.bpe, p.nlod > 1).seg_bytes;
     if (seg)
       info->codec_bytes += lz4_scratch_bytes(cb, per_launch, seg);
   }

@@ -162,12 +162,20 @@ void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric st
 uint32_t compressor_slots();
 // hipHostMalloc on the NUMA node of `device` (default policy if unknown).
 hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
-uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch);
+// Codec launch shapes from the recorded sweep tables (autotune.cpp), by
+// multiscale or not, dtype size, chunk bytes and chunks per compress launch.
+uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);
+bool tune_zstd_chunk_tables(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);
+struct Lz4Choice
+{
+  uint32_t seg_bytes; // 0 = one wavefront per chunk
+  int hash_log;
+};
+Lz4Choice tune_lz4(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);
 uint32_t tune_host_gather_workgroups();
-bool tune_zstd_chunk_tables(uint32_t block);
-uint32_t tune_lz4_segment(uint64_t chunk_bytes, uint64_t chunks_per_launch, int hash_log);
-// LZ4 wavefronts resident at once on the device (LDS-bound by the hash table)
-uint64_t lz4_wave_slots(int hash_log);
+// LZ4 wavefronts resident at once on the device (LDS-bound: hash table and
+// staged segment)
+uint64_t lz4_wave_slots(int hash_log, uint32_t seg_bytes);
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
 int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);

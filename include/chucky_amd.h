@@ -155,6 +155,17 @@ struct shard_sink* chucky_discard_sink_as_shard_sink(struct chucky_discard_sink*
 uint64_t chucky_discard_sink_bytes(const struct chucky_discard_sink* s);
 uint64_t chucky_discard_sink_finalized(const struct chucky_discard_sink* s);
 
+/* The autotuner's codec launch shape for a stream (multiscale != 0: several
+ * levels of detail) whose compress launches hold chunks_per_launch chunks of
+ * chunk_bytes of a bpe-byte dtype (one epoch in the batched pipeline, a batch
+ * of epochs in the plain one); the choice is the nearest row of tables
+ * recorded by a sweep on MI355X. zstd: out[0] = block bytes, out[1] = 1 for
+ * one Huffman table per chunk. lz4: out[0] = segment bytes parsed by one
+ * wavefront (0 = whole chunks), out[1] = hash table log2 entries. 0 on
+ * success. No device is touched. */
+int chucky_tune_codec(enum compression_codec codec, int multiscale, uint32_t bpe, uint64_t chunk_bytes,
+                      uint64_t chunks_per_launch, uint32_t out[2]);
+
 /* ---- kernel-level entry points (device pointers, synchronous) ------------- */
 
 /* Worst-case compressed bytes of one chunk for a GPU codec. */

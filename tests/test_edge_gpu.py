@@ -318,3 +318,15 @@ def test_zero_copy_writes_outlive_the_next_copy(pipeline, codec):
     g, _, cb = geometry(cfg)
     rs, _, _ = run_ref(cfg, data, pieces=2)
     assert_same_shards(sink.shards(), rs, lambda lv: g[lv][0], codec, cb, "deferred sink")
+
+
+@pytest.mark.parametrize("opts,msg", [({"lz4_hash_log": 8}, "lz4_hash_log"),
+                                      ({"lz4_hash_log": 14}, "lz4_hash_log"),
+                                      ({"lz4_segment_bytes": 512}, "lz4_segment_bytes"),
+                                      ({"lz4_segment_bytes": 3000}, "lz4_segment_bytes")])
+def test_create_rejects_bad_lz4_options(opts, msg):
+    cfg = ca.Config([Dim(16, 8, 1), Dim(64, 64, 1), Dim(64, 64, 1)], dtype="u16", codec="lz4")
+    sink = ca.MemSink()
+    with pytest.raises(RuntimeError, match=msg):
+        ca.GpuStream(cfg, sink, **opts)
+    sink.close()

@@ -155,3 +155,52 @@ def test_crc32c_matches_reference_table():
     data = bytes(range(256)) * 5
     from oracle.chucky_oracle import crc32c
     assert ca.crc32c(data) == crc32c(data)
+
+
+def test_autotuner_choice_varies_with_dtype_and_chunk_shape():
+    # rows recorded by the sweep in chucky_amd/tune.py (MI355X): the choice
+    # depends on the dtype size, the chunk bytes and the chunks per launch
+    z = {k: ca.tuned_codec("zstd", *k) for k in [(1, 262144, 256), (2, 524288, 256), (4, 1048576, 256),
+                                                 (2, 524288, 16), (2, 524288, 64)]}
+    assert z[(1, 262144, 256)] == (32768, 0)
+    assert z[(2, 524288, 256)] == (65536, 0)
+    assert z[(4, 1048576, 256)] == (8192, 1)
+    assert z[(2, 524288, 16)] == (8192, 0)
+    assert z[(2, 524288, 64)] == (16384, 0)
+    lz = {k: ca.tuned_codec("lz4", *k) for k in [(1, 262144, 256), (2, 524288, 64), (4, 1048576, 256)]}
+    assert lz[(1, 262144, 256)] == (4096, 9)
+    assert lz[(2, 524288, 64)] == (2048, 9)
+    assert lz[(4, 1048576, 256)] == (1024, 9)
+    # multiscale streams: every level in one launch, the coarse levels' dense
+    # sequences set the tail, so smaller blocks and segments win
+    assert ca.tuned_codec("zstd", 2, 524288, 84, multiscale=True) == (8192, 0)
+    assert ca.tuned_codec("lz4", 2, 524288, 84, multiscale=True) == (1024, 9)
+    # between rows: the nearest on a log scale; a segment never covers the chunk
+    assert ca.tuned_codec("zstd", 2, 600000, 200) == (65536, 0)
+    assert ca.tuned_codec("lz4", 2, 1024, 64)[0] == 0
+    for codec in ("zstd", "lz4"):
+        with pytest.raises(ValueError):
+            ca.tuned_codec(codec, 0, 1024, 1)
+    with pytest.raises(ValueError):
+        ca.tuned_codec("none", 2, 1024, 1)
+
+
+def test_autotuner_tables_match_the_recorded_sweep():
+    # the C++ tables are the choices tune.py derives from the committed sweep
+    from chucky_amd import tune
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rows = tune.load(os.path.join(root, "chucky_amd", "tune_sweep_mi355x.csv"))
+    src = open(os.path.join(root, "chucky_amd", "csrc", "autotune.cpp")).read()
+    for codec, name in (("lz4", "kLz4Table"), ("zstd", "kZstdTable")):
+        body = src[src.index(name + "[] = {"):]
+        body = body[:body.index("};")]
+        got = {tuple(int(x) for x in m[:4]): tuple(int(x) for x in m[4:])
+               for m in re.findall(r"\{ (\d+), (\d+), (\d+), (\d+), (\d+), (\d+) \}", body)}
+        want = {}
+        for (lod, bpe, cb, pl), (o, _, _) in tune.choose([r for r in rows if r["codec"] == codec], codec).items():
+            if codec == "lz4":
+                seg = o["lz4_segment_bytes"]
+                want[(lod, bpe, cb, pl)] = (0 if seg >= cb else seg, o["lz4_hash_log"])
+            else:
+                want[(lod, bpe, cb, pl)] = (o["zstd_block_bytes"], int(o["zstd_chunk_tables"] == 1))
+        assert got == want, codec

@@ -245,8 +245,8 @@ LZ4_SEGMENT_SHAPES = {
 
 
 @pytest.mark.parametrize("shape", sorted(LZ4_SEGMENT_SHAPES))
-@pytest.mark.parametrize("seg,hash_log", [(1024, 10), (4096, 11), (8192, 12), (16384, 12), (32768, 13),
-                                          (1 << 20, 12)])
+@pytest.mark.parametrize("seg,hash_log", [(1024, 9), (2048, 10), (4096, 11), (8192, 12), (16384, 12),
+                                          (32768, 13), (1 << 20, 9), (1 << 20, 12)])
 @pytest.mark.parametrize("data", ["xor", "thirds"])
 @pytest.mark.parametrize("pipeline", [0, 1])
 def test_lz4_segment_sizes_and_hash_logs(shape, seg, hash_log, data, pipeline):
