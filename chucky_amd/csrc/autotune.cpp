@@ -12,7 +12,7 @@
 // `python -m chucky_amd.tune --codec lz4|zstd --out F`, then `--table F`):
 // every dtype size x chunk size x chunks per launch below was streamed once
 // per candidate over two synthetic data kinds (xor+noise volumes, Poisson
-// sensor counts), and the row keeps the candidate with the least time per
+// sensor counts; 4096x2304 sensor frames in 8 MiB chunks), and the row keeps the candidate with the least time per
 // input byte through the compressor plus its output's time over the 55 GB/s
 // device-to-host link. Multiscale streams have rows of their own (every
 // level in one launch; the coarse levels' smooth chunks parse into dense
@@ -41,30 +41,32 @@ struct Lz4Row
 // launch -> segment bytes (0 = whole chunk), hash log; the comment is the
 // sweep's mean rate and ratio for the choice
 const Lz4Row kLz4Table[] = {
-  { 0, 1, 32768, 16, 1024, 9 }, // 28.9 GB/s, 0.996
-  { 0, 1, 32768, 64, 1024, 9 }, // 101.5 GB/s, 0.996
-  { 0, 1, 32768, 256, 1024, 9 }, // 261.1 GB/s, 0.996
-  { 0, 1, 262144, 16, 1024, 9 }, // 137.9 GB/s, 0.996
-  { 0, 1, 262144, 64, 2048, 9 }, // 382.5 GB/s, 0.996
-  { 0, 1, 262144, 256, 4096, 9 }, // 663.7 GB/s, 0.996
-  { 0, 2, 65536, 16, 1024, 9 }, // 35.5 GB/s, 1.012
-  { 0, 2, 65536, 64, 1024, 9 }, // 107.2 GB/s, 1.012
-  { 0, 2, 65536, 256, 1024, 9 }, // 200.7 GB/s, 1.012
-  { 0, 2, 524288, 16, 2048, 9 }, // 137.0 GB/s, 1.013
-  { 0, 2, 524288, 64, 2048, 9 }, // 269.2 GB/s, 1.013
-  { 0, 2, 524288, 256, 2048, 9 }, // 328.6 GB/s, 1.013
-  { 0, 4, 131072, 16, 1024, 9 }, // 42.1 GB/s, 1.072
-  { 0, 4, 131072, 64, 1024, 9 }, // 92.9 GB/s, 1.072
-  { 0, 4, 131072, 256, 1024, 9 }, // 126.6 GB/s, 1.073
-  { 0, 4, 1048576, 16, 1024, 9 }, // 119.5 GB/s, 1.061
-  { 0, 4, 1048576, 64, 1024, 9 }, // 167.8 GB/s, 1.061
-  { 0, 4, 1048576, 256, 1024, 9 }, // 173.4 GB/s, 1.061
-  { 1, 2, 1024, 1365, 0, 9 }, // 22.1 GB/s, 1.168
-  { 1, 2, 1024, 5460, 0, 9 }, // 72.3 GB/s, 1.168
-  { 1, 2, 65536, 85, 1024, 9 }, // 77.1 GB/s, 1.180
-  { 1, 2, 65536, 340, 1024, 9 }, // 162.5 GB/s, 1.180
-  { 1, 2, 524288, 21, 1024, 9 }, // 110.8 GB/s, 1.167
-  { 1, 2, 524288, 84, 1024, 9 }, // 208.3 GB/s, 1.167
+  { 0, 1, 32768, 16, 1024, 9 }, // 28.8 GB/s, 0.996
+  { 0, 1, 32768, 64, 1024, 9 }, // 101.3 GB/s, 0.996
+  { 0, 1, 32768, 256, 1024, 9 }, // 259.6 GB/s, 0.996
+  { 0, 1, 262144, 16, 2048, 9 }, // 136.9 GB/s, 0.996
+  { 0, 1, 262144, 64, 2048, 9 }, // 382.0 GB/s, 0.996
+  { 0, 1, 262144, 256, 4096, 9 }, // 659.7 GB/s, 0.996
+  { 0, 2, 65536, 16, 1024, 9 }, // 35.3 GB/s, 1.012
+  { 0, 2, 65536, 64, 1024, 9 }, // 107.5 GB/s, 1.012
+  { 0, 2, 65536, 256, 1024, 9 }, // 199.5 GB/s, 1.012
+  { 0, 2, 524288, 16, 2048, 9 }, // 136.0 GB/s, 1.013
+  { 0, 2, 524288, 64, 2048, 9 }, // 266.4 GB/s, 1.013
+  { 0, 2, 524288, 256, 2048, 9 }, // 326.3 GB/s, 1.013
+  { 0, 2, 8388608, 144, 2048, 9 }, // 161.6 GB/s, 1.029
+  { 0, 2, 8388608, 288, 1024, 9 }, // 169.2 GB/s, 1.025
+  { 0, 4, 131072, 16, 1024, 9 }, // 41.9 GB/s, 1.072
+  { 0, 4, 131072, 64, 1024, 9 }, // 92.3 GB/s, 1.072
+  { 0, 4, 131072, 256, 1024, 9 }, // 125.8 GB/s, 1.073
+  { 0, 4, 1048576, 16, 1024, 9 }, // 118.9 GB/s, 1.061
+  { 0, 4, 1048576, 64, 1024, 9 }, // 166.2 GB/s, 1.061
+  { 0, 4, 1048576, 256, 1024, 9 }, // 172.1 GB/s, 1.061
+  { 1, 2, 1024, 1365, 0, 9 }, // 21.9 GB/s, 1.168
+  { 1, 2, 1024, 5460, 0, 9 }, // 71.9 GB/s, 1.168
+  { 1, 2, 65536, 85, 1024, 9 }, // 76.4 GB/s, 1.180
+  { 1, 2, 65536, 340, 1024, 9 }, // 162.1 GB/s, 1.180
+  { 1, 2, 524288, 21, 1024, 9 }, // 111.1 GB/s, 1.167
+  { 1, 2, 524288, 84, 1024, 9 }, // 207.7 GB/s, 1.167
 };
 
 struct ZstdRow
@@ -76,29 +78,31 @@ struct ZstdRow
 // Huffman table per chunk
 const ZstdRow kZstdTable[] = {
   { 0, 1, 32768, 16, 8192, 0 }, // 6.3 GB/s, 1.470
-  { 0, 1, 32768, 64, 8192, 0 }, // 23.3 GB/s, 1.470
-  { 0, 1, 32768, 256, 8192, 0 }, // 73.3 GB/s, 1.470
-  { 0, 1, 262144, 16, 8192, 0 }, // 41.8 GB/s, 1.333
-  { 0, 1, 262144, 64, 16384, 0 }, // 113.2 GB/s, 1.339
-  { 0, 1, 262144, 256, 32768, 0 }, // 248.5 GB/s, 1.342
-  { 0, 2, 65536, 16, 8192, 0 }, // 11.0 GB/s, 1.804
-  { 0, 2, 65536, 64, 8192, 0 }, // 38.4 GB/s, 1.803
-  { 0, 2, 65536, 256, 8192, 0 }, // 99.7 GB/s, 1.803
-  { 0, 2, 524288, 16, 8192, 0 }, // 64.7 GB/s, 1.665
-  { 0, 2, 524288, 64, 16384, 0 }, // 160.5 GB/s, 1.675
-  { 0, 2, 524288, 256, 65536, 0 }, // 274.7 GB/s, 1.682
+  { 0, 1, 32768, 64, 8192, 0 }, // 23.4 GB/s, 1.470
+  { 0, 1, 32768, 256, 8192, 0 }, // 73.2 GB/s, 1.470
+  { 0, 1, 262144, 16, 8192, 0 }, // 41.7 GB/s, 1.333
+  { 0, 1, 262144, 64, 16384, 0 }, // 112.3 GB/s, 1.339
+  { 0, 1, 262144, 256, 32768, 0 }, // 247.7 GB/s, 1.342
+  { 0, 2, 65536, 16, 8192, 0 }, // 10.9 GB/s, 1.804
+  { 0, 2, 65536, 64, 8192, 0 }, // 38.5 GB/s, 1.803
+  { 0, 2, 65536, 256, 8192, 0 }, // 99.6 GB/s, 1.803
+  { 0, 2, 524288, 16, 8192, 0 }, // 64.5 GB/s, 1.665
+  { 0, 2, 524288, 64, 16384, 0 }, // 158.9 GB/s, 1.675
+  { 0, 2, 524288, 256, 65536, 0 }, // 274.3 GB/s, 1.682
+  { 0, 2, 8388608, 144, 65536, 0 }, // 411.6 GB/s, 2.154
+  { 0, 2, 8388608, 288, 131072, 0 }, // 450.7 GB/s, 2.155
   { 0, 4, 131072, 16, 8192, 0 }, // 8.2 GB/s, 2.329
-  { 0, 4, 131072, 64, 8192, 0 }, // 29.4 GB/s, 2.309
+  { 0, 4, 131072, 64, 8192, 0 }, // 29.3 GB/s, 2.309
   { 0, 4, 131072, 256, 8192, 1 }, // 55.6 GB/s, 2.336
-  { 0, 4, 1048576, 16, 8192, 0 }, // 56.2 GB/s, 2.185
-  { 0, 4, 1048576, 64, 8192, 0 }, // 67.9 GB/s, 2.184
-  { 0, 4, 1048576, 256, 8192, 1 }, // 99.6 GB/s, 2.211
+  { 0, 4, 1048576, 16, 8192, 0 }, // 55.7 GB/s, 2.185
+  { 0, 4, 1048576, 64, 8192, 0 }, // 67.8 GB/s, 2.184
+  { 0, 4, 1048576, 256, 8192, 1 }, // 99.4 GB/s, 2.211
   { 1, 2, 1024, 1365, 8192, 0 }, // 11.8 GB/s, 2.116
-  { 1, 2, 1024, 5460, 16384, 0 }, // 20.9 GB/s, 2.116
+  { 1, 2, 1024, 5460, 16384, 0 }, // 20.8 GB/s, 2.116
   { 1, 2, 65536, 85, 8192, 0 }, // 11.8 GB/s, 2.131
-  { 1, 2, 65536, 340, 8192, 0 }, // 38.2 GB/s, 2.131
-  { 1, 2, 524288, 21, 8192, 0 }, // 25.5 GB/s, 1.993
-  { 1, 2, 524288, 84, 8192, 0 }, // 65.6 GB/s, 1.993
+  { 1, 2, 65536, 340, 8192, 0 }, // 38.3 GB/s, 2.131
+  { 1, 2, 524288, 21, 8192, 0 }, // 25.0 GB/s, 1.993
+  { 1, 2, 524288, 84, 8192, 0 }, // 64.8 GB/s, 1.993
 };
 
 template<typename Row, size_t N>

@@ -1,10 +1,48 @@
 #include "host_copy.hpp"
 
+#include <emmintrin.h>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
 namespace chucky {
+
+namespace {
+
+// Copy into pinned staging with non-temporal stores: the staging bytes are
+// read next by a DMA engine, not by this CPU, so they need not be cached, and
+// streaming stores skip the read-for-ownership that ordinary stores make,
+// taking host-memory traffic from three times the bytes to two. The fence
+// drains the write-combining buffers before the piece counts as copied.
+void stream_copy(uint8_t* dst, const uint8_t* src, size_t n)
+{
+  constexpr size_t kMinStream = 64u << 10;
+  if (n < kMinStream) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t head = (16 - ((uintptr_t)dst & 15)) & 15;
+  std::memcpy(dst, src, head);
+  dst += head;
+  src += head;
+  n -= head;
+  const size_t body = n & ~(size_t)63;
+  for (size_t i = 0; i < body; i += 64) {
+    const __m128i a = _mm_loadu_si128((const __m128i*)(src + i));
+    const __m128i b = _mm_loadu_si128((const __m128i*)(src + i + 16));
+    const __m128i c = _mm_loadu_si128((const __m128i*)(src + i + 32));
+    const __m128i d = _mm_loadu_si128((const __m128i*)(src + i + 48));
+    _mm_stream_si128((__m128i*)(dst + i), a);
+    _mm_stream_si128((__m128i*)(dst + i + 16), b);
+    _mm_stream_si128((__m128i*)(dst + i + 32), c);
+    _mm_stream_si128((__m128i*)(dst + i + 48), d);
+  }
+  std::memcpy(dst + body, src + body, n - body);
+  _mm_sfence();
+}
+
+} // namespace
 
 CopyPool::CopyPool(unsigned threads)
 {
@@ -30,7 +68,7 @@ void CopyPool::work()
     const size_t off = next_.fetch_add(kPiece);
     if (off >= n_)
       return;
-    std::memcpy(dst_ + off, src_ + off, std::min(kPiece, n_ - off));
+    stream_copy(dst_ + off, src_ + off, std::min(kPiece, n_ - off));
   }
 }
 
@@ -55,7 +93,7 @@ void CopyPool::start(void* dst, const void* src, size_t n)
 {
   wait();
   if (workers_.empty() || n < 2 * kPiece) {
-    std::memcpy(dst, src, n);
+    stream_copy((uint8_t*)dst, (const uint8_t*)src, n);
     return;
   }
   {

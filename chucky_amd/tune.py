@@ -7,8 +7,8 @@
 
 The sweep streams synthetic volumes through the plain pipeline (device-
 resident input, discard sink) for every dtype size x chunk shape x chunks per
-compress launch below, and multiscale volumes for the LOD chunk shapes, once
-per candidate setting, and records the compress
+compress launch below, multiscale volumes for the LOD chunk shapes and
+sensor frames at their own geometry, once per candidate setting, and records the compress
 stage's event-timed milliseconds per launch and its compression ratio. --table
 turns recorded sweeps into the C++ rows of the tables: for each measured case
 (rates and ratios averaged over the data kinds) the candidate with the least
@@ -32,6 +32,9 @@ LAUNCHES = (((4, 4), 1), ((4, 4), 4), ((8, 8), 4))
 # into dense sequences, so their blocks set the launch's tail
 LOD_CHUNKS = ((64, 64, 64), (32, 32, 32), (8, 8, 8))
 LOD_EPOCHS = (1, 4)
+# sensor frames: 4096x2304 u16, 64x256x256 chunks (8 MiB), one 1.2 GB epoch
+# of 144 chunks a launch (or two), Poisson counts synthesized on the device
+SENSOR_EPOCHS = (1, 2)
 LZ4_CANDIDATES = [(seg, h) for seg in (1024, 2048, 4096, 8192, 16384, 1 << 30) for h in (9, 10, 11, 12)]
 ZSTD_CANDIDATES = [(b, t) for b in (8192, 16384, 32768, 65536, 131072) for t in (2, 1)]
 LINK_GBPS = 55.0
@@ -53,12 +56,15 @@ def volume(kind, shape, dtype, seed=7):
     return v.astype(np.float32)
 
 
-def run_candidates(codec, cfg, host, option_sets, case, steps=3, warmup=1):
-    """One row per option set: compress ms per launch, GB/s and ratio."""
+def run_candidates(codec, cfg, host, option_sets, case, steps=3, warmup=1, dev=None):
+    """One row per option set: compress ms per launch, GB/s and ratio.
+    `host` is the input, or (with `dev`, the input already on the device) an
+    array of its dtype and size only."""
     import chucky_amd as ca
     info = ca.memory_estimate(cfg)
     per_launch = info.total_chunks * info.epochs_per_batch
-    dev = ca.DeviceBuffer.from_host(host)
+    if dev is None:
+        dev = ca.DeviceBuffer.from_host(host)
     rows = []
     for opts in option_sets:
         sink = ca.DiscardSink()
@@ -105,6 +111,21 @@ def measure_lod(codec, chunk, K, option_sets, kind, n=256):
     return run_candidates(codec, cfg, volume(kind, (n, n, n), "u16"), option_sets, case)
 
 
+def measure_sensor(codec, K, option_sets, height=4096, width=2304):
+    import ctypes as C
+    import chucky_amd as ca
+    from chucky_amd.api import Config, Dim
+    frames = 64 * 2 * K
+    cfg = Config([Dim(frames, 64, 1), Dim(height, 256, 8), Dim(width, 256, 9)], dtype="u16", codec=codec,
+                 epochs_per_batch=K, buffer_capacity_bytes=64 << 20)
+    dev = ca.DeviceBuffer(frames * height * width * 2)
+    if ca.lib.chucky_synth_sensor_frames(dev.ptr, frames, height, width, 7, 100.0, C.POINTER(C.c_float)(), 0):
+        raise RuntimeError("frame synthesis failed: " + ca.last_error())
+    shape = np.empty((frames, height, width), dtype=np.uint16)  # size and dtype only
+    case = dict(dtype="u16", chunk=[64, 256, 256], chunk_bytes=64 * 256 * 256 * 2, data="sensor", lod=0)
+    return run_candidates(codec, cfg, shape, option_sets, case, dev=dev)
+
+
 def sweep(codec):
     kinds = ("xor", "sensor")
     cands = ([dict(lz4_segment_bytes=s, lz4_hash_log=h) for s, h in LZ4_CANDIDATES] if codec == "lz4" else
@@ -119,6 +140,8 @@ def sweep(codec):
         for K in LOD_EPOCHS:
             for kind in kinds:
                 out += measure_lod(codec, chunk, K, cands, kind)
+    for K in SENSOR_EPOCHS:
+        out += measure_sensor(codec, K, cands)
     return out
 
 
