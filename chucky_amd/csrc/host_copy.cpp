@@ -65,10 +65,10 @@ CopyPool::~CopyPool()
 void CopyPool::work()
 {
   for (;;) {
-    const size_t off = next_.fetch_add(kPiece);
+    const size_t off = next_.fetch_add(piece_);
     if (off >= n_)
       return;
-    stream_copy(dst_ + off, src_ + off, std::min(kPiece, n_ - off));
+    stream_copy(dst_ + off, src_ + off, std::min(piece_, n_ - off));
   }
 }
 
@@ -92,12 +92,14 @@ void CopyPool::run()
 void CopyPool::start(void* dst, const void* src, size_t n)
 {
   wait();
-  if (workers_.empty() || n < 2 * kPiece) {
+  if (workers_.empty() || n < 2 * kMinPiece) {
     stream_copy((uint8_t*)dst, (const uint8_t*)src, n);
     return;
   }
   {
     std::lock_guard<std::mutex> lk(mu_);
+    const size_t per = n / (2 * (workers_.size() + 1));
+    piece_ = std::min(kMaxPiece, std::max(kMinPiece, per & ~(size_t)4095));
     dst_ = (uint8_t*)dst;
     src_ = (const uint8_t*)src;
     n_ = n;

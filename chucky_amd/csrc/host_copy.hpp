@@ -35,7 +35,10 @@ private:
   void run();
   void work();
 
-  static constexpr size_t kPiece = 2u << 20; // bytes a thread takes at a time
+  // bytes a thread takes at a time: about two pieces per thread, so that
+  // every thread has work, within [kMinPiece, kMaxPiece]
+  static constexpr size_t kMinPiece = 256u << 10, kMaxPiece = 2u << 20;
+  size_t piece_ = kMaxPiece;
 
   std::vector<std::thread> workers_;
   std::mutex mu_;

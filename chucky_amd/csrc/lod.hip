@@ -973,6 +973,12 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
   // read it; the rest of the pass works on the Morton levels
   if (hipEventRecord(s->ev_linear_free, st) != hipSuccess)
     return 1;
+  if (s->lod_src_free) {
+    // a staging buffer read in place: free for the next host copy
+    if (hipEventRecord(s->lod_src_free, st) != hipSuccess)
+      return 1;
+    s->lod_src_free = nullptr;
+  }
   int last_reduce = -1;
   for (int l = 0; l + 1 < p.nlod; ++l)
     if (p.levels[l + 1].lod_nelem)

@@ -54,7 +54,11 @@ constexpr uint32_t kMaxZip = 2;
 struct BatchedState
 {
   hipStream_t st_agg = nullptr;
-  hipStream_t st_copy = nullptr;
+  // DMA copies of host slot h go on st_copy[h], and nothing else does: a
+  // copy-only stream carries no event markers, which on a hardware queue
+  // shared with a compression stream would wait behind its kernels; a
+  // copy's completion is its stream's (hipStreamQuery / Synchronize)
+  std::vector<hipStream_t> st_copy;
   // Compression queues: when one epoch's blocks fill only part of the
   // compressor's slots, successive epochs compress side by side on NZ queues,
   // each with its own work counter and share of the worker scratch.
@@ -95,13 +99,18 @@ struct BatchedState
     uint64_t epoch;
     uint32_t pool_slot, host_slot, mask;
     hipEvent_t done;   // aggregate graph finished
-    hipEvent_t copied; // DMA copies finished (DMA mode), else null
-    hipEvent_t c0, c1; // compress graph start / end (timing events)
+    bool copied;       // DMA copies queued (DMA mode) on the host slot's copy stream
+    hipEvent_t c0, c1; // compress launch start / end (timing events)
     std::chrono::steady_clock::time_point t0;
+    double bytes01 = 0; // compressor input bytes between c0 and c1
+    // split epochs: the second compress launch (level 0) and its input bytes
+    hipEvent_t c2 = nullptr, c3 = nullptr;
+    double bytes23 = 0;
     // split epochs: level 0 (mask_a) is compressed, gathered and copied on
     // its own, beside the coarse levels (done / copied)
     uint32_t mask_a = 0;
-    hipEvent_t done_a = nullptr, copied_a = nullptr;
+    hipEvent_t done_a = nullptr;
+    bool copied_a = false;
   };
   std::deque<Ticket> inflight;
   uint64_t next_epoch = 0;
@@ -187,7 +196,9 @@ hipEvent_t take_timing_event(BatchedState* b)
   return e;
 }
 
-int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
+// Chunks [0, n) of the epoch in pool slot `slot` (the captured graphs take
+// every chunk of the slot, n = total_chunks).
+int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st, uint64_t n)
 {
   const BatchedState* b = s->batched;
   const uint32_t z = slot % b->NZ;
@@ -195,16 +206,27 @@ int launch_compress(tile_stream_gpu* s, uint32_t slot, hipStream_t st)
   const uint64_t T = s->plan.total_chunks;
   if (codec.id == CODEC_LZ4_NON_STANDARD)
     BCHECK(launch_lz4(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
-                      s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
+                      s->chunk_bytes, n, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
                       s->d_seg_sizes + (uint64_t)slot * T * s->nseg, s->lz4_hash_log, s->lz4_seg,
                       b->d_lz4 ? b->d_lz4 + z * b->lz4_stride : s->d_lz4_scratch, st));
   else if (codec.id == CODEC_ZSTD)
     BCHECK(launch_zstd(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes, s->chunk_stride_bytes,
-                       s->chunk_bytes, T, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
+                       s->chunk_bytes, n, s->d_comp + (uint64_t)slot * T * s->chunk_slot, s->chunk_slot,
                        s->d_seg_sizes + (uint64_t)slot * T * s->nseg, codec.level, s->zblock,
                        (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers), b->zip_workers,
                        b->d_queues + 16 * z, s->d_ztables, st));
   return 0;
+}
+
+// End of the chunks of the levels in `mask` in an epoch's pool slot: levels
+// are stored one after another, level 0 first.
+uint64_t active_chunk_end(const Plan& p, uint32_t mask)
+{
+  int top = 0;
+  for (int lv = 0; lv < p.nlod; ++lv)
+    if (mask & (1u << lv))
+      top = lv;
+  return top + 1 < p.nlod ? p.levels[top + 1].chunk_offset : p.total_chunks;
 }
 
 // zstd on chunks [c0, c1) of pool slot `slot`, with compression queue z's
@@ -335,7 +357,8 @@ int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t, b
       if (mask & (1u << lv))
         bytes += H.h_offsets[lv][s->plan.levels[lv].agg.covering_count];
     if (bytes)
-      BCHECK(hipMemcpyAsync(H.h_agg + base, H.d_stage + base, bytes, hipMemcpyDeviceToHost, b->st_copy));
+      BCHECK(hipMemcpyAsync(H.h_agg + base, H.d_stage + base, bytes, hipMemcpyDeviceToHost,
+                            b->st_copy[t.host_slot]));
   } else {
     for (int lv = 0; lv < s->plan.nlod; ++lv) {
       if (!(mask & (1u << lv)))
@@ -344,16 +367,10 @@ int issue_copies(tile_stream_gpu* s, BatchedState* b, BatchedState::Ticket& t, b
       const uint64_t bytes = H.h_offsets[lv][C];
       if (bytes)
         BCHECK(hipMemcpyAsync(H.h_agg + H.level_base[lv], H.d_stage + H.level_base[lv], bytes,
-                              hipMemcpyDeviceToHost, b->st_copy));
+                              hipMemcpyDeviceToHost, b->st_copy[t.host_slot]));
     }
   }
-  hipEvent_t& ev = part_a ? t.copied_a : t.copied;
-  ev = take_event(b);
-  if (!ev) {
-    set_error("batched: event creation failed");
-    return 1;
-  }
-  BCHECK(hipEventRecord(ev, b->st_copy));
+  (part_a ? t.copied_a : t.copied) = true;
   return 0;
 }
 
@@ -392,10 +409,13 @@ void copier_loop(tile_stream_gpu* s, BatchedState* b)
       // the submitting thread reports it at its next delivery; no further copies
       b->copy_error = std::string("batched: copy thread: ") +
                       (e != hipSuccess ? hipGetErrorString(e) : chucky_last_error());
+      b->cv.notify_all();
       while (!b->stop)
         b->cv.wait(lk);
       return;
     }
+    // a delivery may be waiting for this copy to be queued
+    b->cv.notify_all();
   }
 }
 
@@ -415,19 +435,25 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
         return 1;
       }
       if (t.copied && (!t.mask_a || t.copied_a)) {
-        hipError_t q = hipEventQuery(t.copied);
-        if (q == hipSuccess && t.mask_a)
-          q = hipEventQuery(t.copied_a);
+        // every copy of the ticket is queued on its host slot's stream, and
+        // only those (the slot is not reused before this delivery)
+        const hipStream_t cs = b->st_copy[t.host_slot];
+        const hipError_t q = hipStreamQuery(cs);
         if (q == hipSuccess)
           break;
         if (q != hipErrorNotReady)
           BCHECK(q);
+        if (!block)
+          return 2;
+        lk.unlock();
+        BCHECK(hipStreamSynchronize(cs));
+        lk.lock();
+        continue;
       }
       if (!block)
         return 2;
-      lk.unlock();
-      std::this_thread::yield();
-      lk.lock();
+      // the copier thread signals once it has queued a copy (or failed)
+      b->cv.wait(lk);
     }
     // (the ticket keeps its copy events until it leaves the queue: the
     // copier thread takes a ticket without them for one still to copy)
@@ -440,17 +466,17 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
     BCHECK(q);
   }
   auto& H = b->host[t.host_slot];
-  if (t.c0 && t.c1) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, t.c0, t.c1) == hipSuccess)
-      metric_add(s->metrics.compress, ms, (double)s->plan.total_chunks * s->chunk_bytes, 0);
-  }
+  // one sample per compress launch: its input bytes and event-timed duration
+  float ms = 0;
+  if (t.c0 && t.c1 && hipEventElapsedTime(&ms, t.c0, t.c1) == hipSuccess)
+    metric_add(s->metrics.compress, ms, t.bytes01, 0);
+  if (t.c2 && t.c3 && hipEventElapsedTime(&ms, t.c2, t.c3) == hipSuccess)
+    metric_add(s->metrics.compress, ms, t.bytes23, 0);
   {
     std::lock_guard<std::recursive_mutex> lk(b->mu);
-    if (t.c0)
-      b->timing_pool.push_back(t.c0);
-    if (t.c1)
-      b->timing_pool.push_back(t.c1);
+    for (hipEvent_t e : { t.c0, t.c1, t.c2, t.c3 })
+      if (e)
+        b->timing_pool.push_back(e);
   }
   scatter_collect_ready(s);
   const auto t1 = std::chrono::steady_clock::now();
@@ -473,9 +499,8 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
   s->epochs_written++;
   std::lock_guard<std::recursive_mutex> lk(b->mu);
   b->event_pool.push_back(t.done);
-  for (hipEvent_t e : { t.copied, t.copied_a, t.done_a })
-    if (e)
-      b->event_pool.push_back(e);
+  if (t.done_a)
+    b->event_pool.push_back(t.done_a);
   b->inflight.pop_front();
   return 0;
 }
@@ -519,7 +544,9 @@ int batched_init(tile_stream_gpu* s)
   } else {
     BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
   }
-  BCHECK(hipStreamCreateWithFlags(&b->st_copy, hipStreamNonBlocking));
+  b->st_copy.assign(b->R, nullptr);
+  for (auto& c : b->st_copy)
+    BCHECK(hipStreamCreateWithFlags(&c, hipStreamNonBlocking));
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
     return 1;
   if (b->NZ > 1 && b->lz4_stride && dev_alloc(s, &b->d_lz4, b->NZ * b->lz4_stride))
@@ -580,7 +607,8 @@ int batched_init(tile_stream_gpu* s)
     BCHECK(hipEventCreateWithFlags(&P.ev_gathered, hipEventDisableTiming));
     BCHECK(hipEventRecord(P.ev_gathered, s->st_comp));
     if (b->graphs && p.config.codec.id != CODEC_NONE &&
-        capture(s->st_comp, &P.compress_exec, [&](hipStream_t st) { return launch_compress(s, ps, st); }))
+        capture(s->st_comp, &P.compress_exec,
+                [&](hipStream_t st) { return launch_compress(s, ps, st, p.total_chunks); }))
       return 1;
   }
   if (b->dma)
@@ -618,16 +646,13 @@ void batched_free(tile_stream_gpu* s)
   }
   for (auto& kv : b->agg_exec)
     (void)hipGraphExecDestroy(kv.second);
-  if (b->st_copy)
-    (void)hipStreamSynchronize(b->st_copy);
+  for (auto c : b->st_copy)
+    if (c)
+      (void)hipStreamSynchronize(c);
   for (auto& t : b->inflight) {
     (void)hipEventDestroy(t.done);
-    if (t.copied)
-      (void)hipEventDestroy(t.copied);
     if (t.done_a)
       (void)hipEventDestroy(t.done_a);
-    if (t.copied_a)
-      (void)hipEventDestroy(t.copied_a);
     if (t.c0)
       (void)hipEventDestroy(t.c0);
     if (t.c1)
@@ -639,8 +664,9 @@ void batched_free(tile_stream_gpu* s)
     (void)hipEventDestroy(e);
   if (b->st_agg)
     (void)hipStreamDestroy(b->st_agg);
-  if (b->st_copy)
-    (void)hipStreamDestroy(b->st_copy);
+  for (auto c : b->st_copy)
+    if (c)
+      (void)hipStreamDestroy(c);
   for (auto z : b->st_zip)
     if (z)
       (void)hipStreamDestroy(z);
@@ -679,30 +705,38 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       BCHECK(hipStreamWaitEvent(z, P.ev_filled, 0));
       BCHECK(hipStreamWaitEvent(z, P.ev_gathered, 0));
     }
+    const uint64_t n1 = active_chunk_end(p, mask);
     hipEvent_t c0 = take_timing_event(b), c1 = take_timing_event(b);
+    hipEvent_t c2 = take_timing_event(b), c3 = take_timing_event(b);
     hipEvent_t done_a = take_event(b), done = take_event(b);
-    if (!c0 || !c1 || !done_a || !done) {
+    if (!c0 || !c1 || !c2 || !c3 || !done_a || !done) {
       set_error("batched: event creation failed");
       return 1;
     }
     BCHECK(hipEventRecord(c0, zb));
-    if (launch_zstd_range(s, pool_slot, pool_slot % b->NZ, n0, p.total_chunks, zb) ||
-        launch_zstd_range(s, pool_slot, (pool_slot + 1) % b->NZ, 0, n0, za))
+    if (launch_zstd_range(s, pool_slot, pool_slot % b->NZ, n0, n1, zb))
       return 1;
+    BCHECK(hipEventRecord(c1, zb));
+    BCHECK(hipEventRecord(c2, za));
+    if (launch_zstd_range(s, pool_slot, (pool_slot + 1) % b->NZ, 0, n0, za))
+      return 1;
+    BCHECK(hipEventRecord(c3, za));
     BCHECK(hipEventRecord(P.ev_compressed, za));
     if (launch_aggregate(s, b, pool_slot, hs, 1u, za, 0))
       return 1;
     BCHECK(hipEventRecord(done_a, za));
-    // the compress time spans both launches
     BCHECK(hipStreamWaitEvent(zb, P.ev_compressed, 0));
-    BCHECK(hipEventRecord(c1, zb));
     if (launch_aggregate(s, b, pool_slot, hs, mask & ~1u, zb, H.level_base[1]))
       return 1;
     BCHECK(hipStreamWaitEvent(zb, done_a, 0));
     BCHECK(hipEventRecord(P.ev_gathered, zb));
     BCHECK(hipEventRecord(done, zb));
-    BatchedState::Ticket t{ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
+    BatchedState::Ticket t{ ticket, pool_slot, hs, mask, done, false, c0, c1,
                             std::chrono::steady_clock::now() };
+    t.bytes01 = (double)(n1 - n0) * s->chunk_bytes;
+    t.c2 = c2;
+    t.c3 = c3;
+    t.bytes23 = (double)n0 * s->chunk_bytes;
     t.mask_a = 1u;
     t.done_a = done_a;
     {
@@ -729,6 +763,10 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
   }
   BCHECK(hipStreamWaitEvent(zs, P.ev_gathered, 0));
   hipEvent_t c0 = nullptr, c1 = nullptr;
+  // direct launches compress the active levels' chunks only (levels are
+  // stored in order, so they are the chunks below the last active level's
+  // end); the captured graph takes every chunk of the slot
+  const uint64_t n_comp = b->graphs ? p.total_chunks : active_chunk_end(p, mask);
   if (s->plan.config.codec.id != CODEC_NONE) {
     c0 = take_timing_event(b);
     c1 = take_timing_event(b);
@@ -739,7 +777,7 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
     BCHECK(hipEventRecord(c0, zs));
     if (b->graphs)
       BCHECK(hipGraphLaunch(P.compress_exec, zs));
-    else if (launch_compress(s, pool_slot, zs))
+    else if (launch_compress(s, pool_slot, zs, n_comp))
       return 1;
     BCHECK(hipEventRecord(c1, zs));
   }
@@ -771,8 +809,8 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
   BCHECK(hipEventRecord(done, as));
   {
     std::lock_guard<std::recursive_mutex> lk(b->mu);
-    b->inflight.push_back({ ticket, pool_slot, hs, mask, done, nullptr, c0, c1,
-                            std::chrono::steady_clock::now() });
+    b->inflight.push_back({ ticket, pool_slot, hs, mask, done, false, c0, c1,
+                            std::chrono::steady_clock::now(), (double)n_comp * s->chunk_bytes });
   }
   b->cv.notify_all();
   // opportunistic in-order delivery of whatever has finished
@@ -811,7 +849,8 @@ int batched_drain(tile_stream_gpu* s)
     BCHECK(hipStreamSynchronize(z));
   if (b->st_agg)
     BCHECK(hipStreamSynchronize(b->st_agg));
-  BCHECK(hipStreamSynchronize(b->st_copy));
+  for (auto c : b->st_copy)
+    BCHECK(hipStreamSynchronize(c));
   scatter_collect_ready(s);
   return 0;
 }

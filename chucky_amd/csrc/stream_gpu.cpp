@@ -402,26 +402,60 @@ int linear_prepare(tile_stream_gpu* s)
   return 0;
 }
 
-int dispatch_stage(tile_stream_gpu* s)
+int end_epoch(tile_stream_gpu* s);
+
+// Queues the launches of an epoch whose end was deferred (epoch_pending).
+int settle_epoch(tile_stream_gpu* s)
+{
+  if (!s->epoch_pending)
+    return 0;
+  s->epoch_pending = false;
+  return end_epoch(s);
+}
+
+// The staged piece, in two steps. dispatch_h2d: the H2D copy into the
+// device staging slot. dispatch_rest: the launches of a finished epoch that
+// waited for that copy to be queued, then the piece's move into the epoch:
+// the chunk scatter, or for multiscale streams a device copy into the epoch
+// buffer -- unless the piece is a whole epoch, which the LOD gather then
+// reads straight from the staging slot.
+int dispatch_h2d(tile_stream_gpu* s)
 {
   const size_t bytes = s->stage_fill;
   if (bytes == 0)
     return 0;
   const int i = s->stage_cur;
+  HIPCHECK(hipMemcpyAsync(s->d_stage[i], s->h_stage[i], bytes, hipMemcpyHostToDevice, s->st_h2d));
+  HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
+  return 0;
+}
+
+int dispatch_rest(tile_stream_gpu* s)
+{
+  const size_t bytes = s->stage_fill;
+  if (settle_epoch(s))
+    return 1;
+  if (bytes == 0)
+    return 0;
+  const int i = s->stage_cur;
   const uint64_t n = bytes / s->plan.bpe;
   const uint64_t E = epoch_elements(s);
+  HIPCHECK(hipStreamWaitEvent(s->st_comp, s->ev_h2d[i], 0));
+  bool held = false;
   if (s->plan.enable_multiscale) {
-    if (linear_prepare(s))
-      return 1;
-    HIPCHECK(hipStreamWaitEvent(s->st_h2d, s->ev_linear_free, 0));
-    HIPCHECK(hipMemcpyAsync(s->d_linear + (s->cursor % E) * s->plan.bpe, s->h_stage[i], bytes,
-                            hipMemcpyHostToDevice, s->st_h2d));
-    HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
-    HIPCHECK(hipStreamWaitEvent(s->st_comp, s->ev_h2d[i], 0));
+    const uint64_t off = s->cursor % E;
+    if (off == 0 && n == E) {
+      // the gather records the slot's release once it has read it
+      s->lod_src = s->d_stage[i];
+      s->lod_src_free = s->ev_stage_free[i];
+      held = true;
+    } else {
+      if (linear_prepare(s))
+        return 1;
+      HIPCHECK(hipMemcpyAsync(s->d_linear + off * s->plan.bpe, s->d_stage[i], bytes, hipMemcpyDeviceToDevice,
+                              s->st_comp));
+    }
   } else {
-    HIPCHECK(hipMemcpyAsync(s->d_stage[i], s->h_stage[i], bytes, hipMemcpyHostToDevice, s->st_h2d));
-    HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
-    HIPCHECK(hipStreamWaitEvent(s->st_comp, s->ev_h2d[i], 0));
     hipEvent_t ta = nullptr;
     if (scatter_begin(s, &ta))
       return 1;
@@ -430,10 +464,11 @@ int dispatch_stage(tile_stream_gpu* s)
     if (scatter_end(s, ta, (double)bytes))
       return 1;
   }
-  HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
+  if (!held)
+    HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
   s->cursor += n;
   s->stage_fill = 0;
-  s->stage_cur ^= 1;
+  s->stage_cur = (s->stage_cur + 1) % tile_stream_gpu::kStageSlots;
   s->metrics.h2d.input_bytes += (double)bytes;
   s->metrics.h2d.count++;
   return 0;
@@ -512,6 +547,8 @@ int end_epoch(tile_stream_gpu* s)
   return 0;
 }
 
+int dispatch_stage(tile_stream_gpu* s) { return dispatch_h2d(s) || dispatch_rest(s); }
+
 int flush_all(tile_stream_gpu* s)
 {
   if (dispatch_stage(s))
@@ -569,6 +606,36 @@ void prefetch_drop(tile_stream_gpu* s)
     s->copy_pool->wait();
     s->pf_n = 0;
   }
+}
+
+// Starts the host copy of the piece the append loop takes next: the rest of
+// the current epoch segment (`seg_left` bytes at `next`), else the start of
+// the next segment. Called between a piece's H2D copy and its launches, when
+// the cursor has not yet moved past the piece (stage_fill elements pending).
+// The next slot was released by an earlier piece's launches, all queued.
+void prefetch_next(tile_stream_gpu* s, const uint8_t* next, uint64_t seg_left, const uint8_t* end)
+{
+  if (!s->copy_pool || s->pf_n)
+    return;
+  const size_t bpe = s->plan.bpe;
+  uint64_t nb = seg_left;
+  if (nb == 0) {
+    const uint64_t E = epoch_elements(s);
+    const uint64_t maxc = s->plan.max_cursor_elements;
+    const uint64_t cur = s->cursor + s->stage_fill / bpe;
+    uint64_t ne = std::min<uint64_t>(E - cur % E, (uint64_t)(end - next) / bpe);
+    if (maxc)
+      ne = std::min<uint64_t>(ne, maxc > cur ? maxc - cur : 0);
+    nb = ne * bpe;
+  }
+  nb = std::min<uint64_t>(nb, s->stage_cap);
+  const int j = (s->stage_cur + 1) % tile_stream_gpu::kStageSlots;
+  if (!nb || hipEventSynchronize(s->ev_stage_free[j]) != hipSuccess)
+    return;
+  s->copy_pool->start(s->h_stage[j], next, (size_t)nb);
+  s->pf_src = next;
+  s->pf_n = (size_t)nb;
+  s->pf_stage = j;
 }
 
 template<bool DEVICE>
@@ -638,7 +705,7 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
         const auto t0 = std::chrono::steady_clock::now();
         if (s->pf_n && s->pf_src == src + written && s->pf_stage == i && s->stage_fill == 0 &&
             s->pf_n == payload) {
-          // begun before the previous epoch's launches
+          // begun while the previous piece's launches were queued
           s->copy_pool->wait();
           s->pf_n = 0;
         } else {
@@ -652,34 +719,33 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
                    (double)payload, (double)payload);
         s->stage_fill += payload;
         written += payload;
-        if (s->stage_fill == s->stage_cap || written == bytes)
-          if (dispatch_stage(s))
+        if (s->stage_fill == s->stage_cap || written == bytes) {
+          if (dispatch_h2d(s))
             return writer_error_at(src, end);
+          // the next piece's host copy runs on the copy threads while this
+          // thread queues the launches below
+          prefetch_next(s, src + written, bytes - written, end);
+          if (dispatch_rest(s))
+            return writer_error_at(src, end);
+        }
       }
     }
     src += bytes;
     if (s->cursor % E == 0 && s->cursor > 0) {
-      // host appends: the next epoch's first staging copy runs on the copy
-      // threads while this thread issues the finished epoch's work
-      if (!DEVICE && s->copy_pool && s->stage_fill == 0) {
-        uint64_t ne = std::min<uint64_t>(E, (uint64_t)(end - src) / bpe);
-        if (maxc)
-          ne = std::min<uint64_t>(ne, maxc > s->cursor ? maxc - s->cursor : 0);
-        const size_t nb = (size_t)std::min<uint64_t>(ne * bpe, s->stage_cap);
-        if (nb && hipEventSynchronize(s->ev_stage_free[s->stage_cur]) == hipSuccess) {
-          s->copy_pool->start(s->h_stage[s->stage_cur], src, nb);
-          s->pf_src = src;
-          s->pf_n = nb;
-          s->pf_stage = s->stage_cur;
-        }
-      }
-      if (end_epoch(s)) {
+      // host appends with more input: the epoch's launches follow the next
+      // piece's H2D copy (dispatch_stage); otherwise they go now
+      const bool more = !DEVICE && (uint64_t)(end - src) >= bpe && !(maxc && s->cursor >= maxc);
+      if (more) {
+        s->epoch_pending = true;
+      } else if (end_epoch(s)) {
         prefetch_drop(s);
         return writer_error_at(src, end);
       }
     }
   }
   prefetch_drop(s);
+  if (settle_epoch(s))
+    return writer_error_at(src, end);
   return writer_result{ 0, { src, end } };
 }
 
@@ -721,7 +787,7 @@ void free_stream(tile_stream_gpu* s)
   }
   for (hipEvent_t e : s->timing_free)
     (void)hipEventDestroy(e);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < tile_stream_gpu::kStageSlots; ++i) {
     if (s->ev_h2d[i])
       (void)hipEventDestroy(s->ev_h2d[i]);
     if (s->ev_stage_free[i])
@@ -797,10 +863,10 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   // staging
   s->stage_cap = std::max<size_t>(align_up(config->buffer_capacity_bytes, 4096), 4096);
   s->stage_cap -= s->stage_cap % p.bpe;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < tile_stream_gpu::kStageSlots; ++i) {
     if (halloc(s, (void**)&s->h_stage[i], s->stage_cap))
       return 1;
-    if (!p.enable_multiscale && dalloc(s, (void**)&s->d_stage[i], s->stage_cap))
+    if (dalloc(s, (void**)&s->d_stage[i], s->stage_cap))
       return 1;
     HIPCHECK(hipEventCreateWithFlags(&s->ev_h2d[i], hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&s->ev_stage_free[i], hipEventDisableTiming));
@@ -1066,6 +1132,9 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   s->at_capacity = 0;
   std::fill(s->slot_compressed.begin(), s->slot_compressed.end(), (uint8_t)0);
   s->stage_fill = 0;
+  s->epoch_pending = false;
+  s->lod_src = nullptr;
+  s->lod_src_free = nullptr;
   std::fill(std::begin(s->append_counts), std::end(s->append_counts), 0u);
   for (auto& ss : s->shards)
     ss.reset();
@@ -1115,7 +1184,7 @@ int estimate_core(const struct tile_stream_configuration* config, struct tile_st
   const uint64_t csb = p.levels[0].layout.chunk_stride * p.bpe;
   const uint64_t cb = p.levels[0].layout.chunk_elements * p.bpe;
   const uint64_t nchunks = (uint64_t)K * p.total_chunks;
-  info->staging_bytes = 2 * align_up(config->buffer_capacity_bytes, 4096);
+  info->staging_bytes = tile_stream_gpu::kStageSlots * align_up(config->buffer_capacity_bytes, 4096);
   info->chunk_pool_bytes = nchunks * csb;
   info->compressed_pool_bytes = config->codec.id == CODEC_NONE ? 0 : nchunks * p.max_output_size;
   uint64_t agg = 0;

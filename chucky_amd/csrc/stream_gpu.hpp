@@ -34,14 +34,21 @@ struct tile_stream_gpu
 
   hipStream_t st_h2d = nullptr, st_comp = nullptr, st_d2h = nullptr;
 
-  // host staging ring (host-memory appends)
+  // host staging ring (host-memory appends): kStageSlots pinned buffers and
+  // their device twins, so that a piece's host copy overlaps the H2D copy of
+  // the one before and the launches of the epoch before that
+  static constexpr int kStageSlots = 3;
   size_t stage_cap = 0;
-  uint8_t* h_stage[2] = {};
-  uint8_t* d_stage[2] = {};
-  hipEvent_t ev_h2d[2] = {}, ev_stage_free[2] = {};
+  uint8_t* h_stage[kStageSlots] = {};
+  uint8_t* d_stage[kStageSlots] = {};
+  hipEvent_t ev_h2d[kStageSlots] = {}, ev_stage_free[kStageSlots] = {};
   hipEvent_t ev_linear_free = nullptr; // LOD gather of the last epoch has read d_linear
   size_t stage_fill = 0;
   int stage_cur = 0;
+  // host appends: a finished epoch's launches are queued right after the
+  // next piece's H2D copy (so the copy engine moves the next epoch while the
+  // device works on this one); set while that copy is still to be queued
+  bool epoch_pending = false;
   std::unique_ptr<chucky::CopyPool> copy_pool; // staging copy threads, started at the first host append
   // a staging copy begun before an epoch's launches: source, bytes, stage
   const uint8_t* pf_src = nullptr;
@@ -84,7 +91,8 @@ struct tile_stream_gpu
 
   // LOD (multiscale)
   uint8_t* d_linear = nullptr;
-  const uint8_t* lod_src = nullptr; // caller's device epoch read in place of d_linear (one epoch)
+  const uint8_t* lod_src = nullptr; // device epoch read in place of d_linear (caller's buffer or staging)
+  hipEvent_t lod_src_free = nullptr; // recorded by the LOD gather once it has read lod_src, or null
   bool linear_in_lod = false;       // the last epoch was read in place: d_linear lacks its values
   uint8_t* d_lod = nullptr;     // Morton values, all levels packed
   uint8_t* d_accum = nullptr;   // append-dim accumulators, levels >= 1
