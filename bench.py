@@ -411,6 +411,8 @@ def main():
                     help="zstd Huffman tables: 0 = autotuned, 1 = one per chunk, 2 = one per block")
     ap.add_argument("--batched-launch", type=int, default=0,
                     help="batched pipeline per-epoch work: 0 = autotuned, 1 = captured graphs, 2 = direct")
+    ap.add_argument("--pass-timing", type=int, default=0,
+                    help="event timing of each scatter / LOD launch: 0/1 = on, 2 = off")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device copy of BYTES (the vector scatter kernel over an "
                          "identity layout), the reference for counter-based HBM traffic")
@@ -430,7 +432,7 @@ def main():
     import chucky_amd as ca
     host_input = not args.device_input
     options = {"zstd_block_bytes": args.zstd_block, "batched_launch": args.batched_launch,
-               "zstd_chunk_tables": args.zstd_chunk_tables}
+               "zstd_chunk_tables": args.zstd_chunk_tables, "pass_timing": args.pass_timing}
     cfg, shape, desc = workload(args.workload, args.frames)
     data, host = make_input(args.workload, shape, True)
     h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)

@@ -969,12 +969,9 @@ int run_epoch_t(tile_stream_gpu* s, uint32_t e, uint32_t* out_mask)
                            s->d_gather_fixed, n0, p.fixed_count, st, ta, tb) != hipSuccess)
     return 1;
   timing_add(s, ta, tb, &stream_metrics::lod_gather, nb * n0 * bpe, nb * n0 * bpe);
-  // the epoch buffer is free for the next epoch's copy once the gather has
-  // read it; the rest of the pass works on the Morton levels
-  if (hipEventRecord(s->ev_linear_free, st) != hipSuccess)
-    return 1;
   if (s->lod_src_free) {
-    // a staging buffer read in place: free for the next host copy
+    // a staging buffer read in place: free for the next host copy once the
+    // gather has read it (the rest of the pass works on the Morton levels)
     if (hipEventRecord(s->lod_src_free, st) != hipSuccess)
       return 1;
     s->lod_src_free = nullptr;

@@ -68,6 +68,10 @@ struct BatchedState
   bool graphs = false;
   std::vector<hipStream_t> st_zip;
   uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
+  // zstd split epochs: per compression queue, the sizes of the half-segment
+  // blocks the coarse levels are compressed in (2 u32 per segment)
+  uint32_t* d_sub_sizes = nullptr;
+  uint64_t sub_stride = 0;
   // LZ4 segment scratch, one slice of lz4_stride bytes per compression queue
   uint8_t* d_lz4 = nullptr;
   uint64_t lz4_stride = 0;
@@ -243,6 +247,24 @@ int launch_zstd_range(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0
                      s->d_seg_sizes + ((uint64_t)slot * T + c0) * s->nseg, s->plan.config.codec.level, s->zblock,
                      (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers), b->zip_workers,
                      b->d_queues + 16 * z, nullptr, st));
+  return 0;
+}
+
+// zstd on the coarse chunks [c0, c1) of a split epoch: two wavefronts per
+// block-sized segment when the layout allows (these launches hold few
+// chunks, and coarse blocks are the slowest), else as launch_zstd_range.
+int launch_zstd_coarse(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0, uint64_t c1, hipStream_t st)
+{
+  const BatchedState* b = s->batched;
+  if (!b->d_sub_sizes || c1 <= c0)
+    return launch_zstd_range(s, slot, z, c0, c1, st);
+  const uint64_t T = s->plan.total_chunks;
+  BCHECK(launch_zstd_halves(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes + c0 * s->chunk_stride_bytes,
+                            s->chunk_stride_bytes, s->chunk_bytes, c1 - c0,
+                            s->d_comp + ((uint64_t)slot * T + c0) * s->chunk_slot, s->chunk_slot,
+                            s->d_seg_sizes + ((uint64_t)slot * T + c0) * s->nseg, b->d_sub_sizes + z * b->sub_stride,
+                            s->zblock, (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers),
+                            b->zip_workers, b->d_queues + 16 * z, st));
   return 0;
 }
 
@@ -549,6 +571,14 @@ int batched_init(tile_stream_gpu* s)
     BCHECK(hipStreamCreateWithFlags(&c, hipStreamNonBlocking));
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
     return 1;
+  // split epochs (see batched_submit) compress their coarse levels in
+  // half-segment blocks
+  if (b->NZ > 1 && b->dma && !b->graphs && p.nlod > 1 && p.nlod <= kAggMaxLevels &&
+      p.config.codec.id == CODEC_ZSTD && zstd_halves_ok(s->chunk_bytes, s->zblock)) {
+    b->sub_stride = align_up((uint64_t)p.total_chunks * s->nseg * 2, 64);
+    if (dev_alloc(s, &b->d_sub_sizes, b->NZ * b->sub_stride))
+      return 1;
+  }
   if (b->NZ > 1 && b->lz4_stride && dev_alloc(s, &b->d_lz4, b->NZ * b->lz4_stride))
     return 1;
   // the compressors' work counters start at zero and each launch leaves them so
@@ -714,7 +744,7 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       return 1;
     }
     BCHECK(hipEventRecord(c0, zb));
-    if (launch_zstd_range(s, pool_slot, pool_slot % b->NZ, n0, n1, zb))
+    if (launch_zstd_coarse(s, pool_slot, pool_slot % b->NZ, n0, n1, zb))
       return 1;
     BCHECK(hipEventRecord(c1, zb));
     BCHECK(hipEventRecord(c2, za));

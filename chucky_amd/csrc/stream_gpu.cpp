@@ -267,6 +267,9 @@ int aggregate_level(tile_stream_gpu* s, int lv, uint32_t active, const uint32_t*
 // once the queue has passed them.
 int time_begin(tile_stream_gpu* s, hipEvent_t* a)
 {
+  *a = nullptr;
+  if (!s->pass_timing)
+    return 0;
   *a = take_timing(s);
   if (!*a) {
     set_error("event creation failed");
@@ -279,6 +282,8 @@ int time_begin(tile_stream_gpu* s, hipEvent_t* a)
 int time_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,
              double out_bytes)
 {
+  if (!a)
+    return 0;
   hipEvent_t b = take_timing(s);
   if (!b) {
     set_error("event creation failed");
@@ -398,7 +403,6 @@ int linear_prepare(tile_stream_gpu* s)
     return 1;
   }
   s->linear_in_lod = false;
-  HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
   return 0;
 }
 
@@ -512,7 +516,6 @@ int end_epoch(tile_stream_gpu* s)
 {
   uint32_t mask = 1;
   if (s->plan.enable_multiscale) {
-    // (the LOD pass records ev_linear_free once its gather has read the epoch)
     if (lod_run_epoch(s, s->batch_acc, &mask))
       return 1;
   }
@@ -779,8 +782,6 @@ void free_stream(tile_stream_gpu* s)
     (void)hipStreamSynchronize(s->st_h2d);
   if (s->st_d2h)
     (void)hipStreamSynchronize(s->st_d2h);
-  if (s->ev_linear_free)
-    (void)hipEventDestroy(s->ev_linear_free);
   for (auto& t : s->pending_timing) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -844,6 +845,7 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   s->w.flush = gpu_flush;
   if (s->opts.device >= 0)
     HIPCHECK(hipSetDevice(s->opts.device));
+  s->pass_timing = s->opts.pass_timing != 2;
   HIPCHECK(hipGetDevice(&s->device));
   HIPCHECK(hipStreamCreateWithFlags(&s->st_h2d, hipStreamNonBlocking));
   HIPCHECK(hipStreamCreateWithFlags(&s->st_comp, hipStreamNonBlocking));
@@ -857,8 +859,6 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
   s->chunk_bytes = L0.chunk_elements * p.bpe;
   s->epoch_pool_bytes = p.total_chunks * s->chunk_stride_bytes;
 
-  HIPCHECK(hipEventCreateWithFlags(&s->ev_linear_free, hipEventDisableTiming));
-  HIPCHECK(hipEventRecord(s->ev_linear_free, s->st_comp));
 
   // staging
   s->stage_cap = std::max<size_t>(align_up(config->buffer_capacity_bytes, 4096), 4096);
@@ -1022,6 +1022,9 @@ void scatter_collect_ready(tile_stream_gpu* s) { collect_scatter_ready(s); }
 int timing_begin(tile_stream_gpu* s, hipEvent_t* a) { return time_begin(s, a); }
 int timing_pair(tile_stream_gpu* s, hipEvent_t* a, hipEvent_t* b)
 {
+  *a = *b = nullptr;
+  if (!s->pass_timing)
+    return 0;
   *a = take_timing(s);
   *b = take_timing(s);
   if (!*a || !*b) {
@@ -1036,6 +1039,8 @@ int timing_pair(tile_stream_gpu* s, hipEvent_t* a, hipEvent_t* b)
 void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric stream_metrics::*which,
                 double in_bytes, double out_bytes)
 {
+  if (!a || !b)
+    return;
   s->pending_timing.push_back({ a, b, in_bytes, out_bytes, which });
 }
 int timing_end(tile_stream_gpu* s, hipEvent_t a, stream_metric stream_metrics::*which, double in_bytes,

@@ -42,7 +42,6 @@ struct tile_stream_gpu
   uint8_t* h_stage[kStageSlots] = {};
   uint8_t* d_stage[kStageSlots] = {};
   hipEvent_t ev_h2d[kStageSlots] = {}, ev_stage_free[kStageSlots] = {};
-  hipEvent_t ev_linear_free = nullptr; // LOD gather of the last epoch has read d_linear
   size_t stage_fill = 0;
   int stage_cur = 0;
   // host appends: a finished epoch's launches are queued right after the
@@ -134,6 +133,9 @@ struct tile_stream_gpu
     stream_metric stream_metrics::*which;
   };
   std::vector<PendingTiming> pending_timing;
+  // per-pass event timing of the scatter and LOD launches (stream metrics);
+  // off, those launches run back to back without event markers between them
+  bool pass_timing = true;
   std::vector<hipEvent_t> timing_free; // recycled timing events
 
   chucky::BatchedState* batched = nullptr; // non-null: batched pipeline active

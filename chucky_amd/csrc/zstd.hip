@@ -1059,52 +1059,77 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
         pos += kWave * stride;
         continue;
       }
-      // greedy over the window: the first hit, then the first hit at or past
-      // the end of the previous match, and so on
+      // Greedy over the window, wave-parallel: the first hit, then the first
+      // hit at or past the end of the previous match, and so on. Every hit
+      // lane knows its own match end, hence its successor in that chain; the
+      // scalar unit walks the chain (one lane read per match) and the
+      // selected lanes then finish their sequences side by side. A lane whose
+      // forward match reached kLaneExt has an end only the wave extension
+      // knows: the chain stops there, and resumes past its extended end.
+      const uint32_t end_l = p + ml_lane;
+      uint32_t succ = kWave;
+      if (hit && ml_lane < kLaneExt) {
+        const uint32_t sp = stride == 1 ? end_l - pos : (end_l - pos + stride - 1) / stride;
+        const uint64_t rest = sp >= (uint32_t)kWave ? 0ull : mask & (~0ull << sp);
+        if (rest)
+          succ = (uint32_t)(__ffsll((unsigned long long)rest) - 1);
+      }
+      const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
       do {
-        const int first = __ffsll((unsigned long long)mask) - 1;
-        uint32_t mp = pos + (uint32_t)first * stride;
-        // `first` is uniform: lane reads, not LDS shuffles
-        uint32_t mc = mp - (uint32_t)__builtin_amdgcn_readlane((int)dist, first);
-        uint32_t ml = (uint32_t)__builtin_amdgcn_readlane((int)ml_lane, first);
-        if (ml == kLaneExt)
-          ml = wave_match_extend(in, mp, mc, ml, len);
-        uint32_t bx = (uint32_t)__builtin_amdgcn_readlane((int)bx_lane, first);
-        bx = min(bx, mp - anchor);
-        mp -= bx;
-        mc -= bx;
-        ml += bx;
-        if (bx == 8)
-          while (mp > anchor && mc > 0 && in[mp - 1] == in[mc - 1]) {
-            --mp;
-            --mc;
-            ++ml;
-          }
-        const uint32_t ll = mp - anchor;
-        nlits += ll;
-        if (lane == 0)
-          seqs[nseq] = Seq{ ll, ml, mp - mc, mp };
-        ++nseq;
-        anchor = mp + ml;
-        if (anchor >= 2 && anchor - 2 <= last_start) {
-          // position q's hash is at hand when a lane of this window holds q
-          const uint32_t q = anchor - 2, dq = q - pos;
-          uint32_t hq;
-          if (stride == 1 && q >= pos && dq < (uint32_t)kWave)
-            hq = (uint32_t)__builtin_amdgcn_readlane((int)h, (int)dq);
-          else
-            hq = key_hash(ld64u(in + q));
-          if (lane == 0)
-            L.u.table[hq] = (uint16_t)q;
+        uint64_t sel = 0;
+        uint32_t cur = (uint32_t)(__ffsll((unsigned long long)mask) - 1), last = cur;
+        for (uint32_t room = kMaxSeq - nseq; cur < (uint32_t)kWave && room; --room) {
+          sel |= 1ull << cur;
+          last = cur;
+          cur = (uint32_t)__builtin_amdgcn_readlane((int)succ, (int)cur);
+        }
+        const uint32_t pl = pos + last * stride;
+        uint32_t ml_f = ml_lane;
+        if ((uint32_t)__builtin_amdgcn_readlane((int)ml_lane, (int)last) == kLaneExt) {
+          const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)last);
+          const uint32_t ext = wave_match_extend(in, pl, pl - dl, kLaneExt, len);
+          if ((uint32_t)lane == last)
+            ml_f = ext;
+        }
+        // the previous selected match's end (the anchor for the first)
+        const uint64_t before = sel & below;
+        const int prev = before ? 63 - __clzll(before) : 0;
+        const uint32_t prev_lane_end = (uint32_t)__shfl((int)end_l, prev, kWave);
+        const uint32_t prev_end = before ? prev_lane_end : anchor;
+        uint32_t ll = 0, end_sel = 0;
+        if ((sel >> lane) & 1) {
+          uint32_t mp = p, mc = p - dist, ml = ml_f;
+          const uint32_t bx = min(bx_lane, p - prev_end);
+          mp -= bx;
+          mc -= bx;
+          ml += bx;
+          if (bx == 8)
+            while (mp > prev_end && mc > 0 && in[mp - 1] == in[mc - 1]) {
+              --mp;
+              --mc;
+              ++ml;
+            }
+          ll = mp - prev_end;
+          seqs[nseq + (uint32_t)__popcll(before)] = Seq{ ll, ml, mp - mc, mp };
+          end_sel = mp + ml;
+          // the position two bytes before the match end goes into the table
+          if (end_sel >= 2 && end_sel - 2 <= last_start)
+            L.u.table[key_hash(ld64u(in + end_sel - 2))] = (uint16_t)(end_sel - 2);
         }
         __builtin_amdgcn_wave_barrier();
-        // lanes whose position lies inside the match are spent
+        nlits += wave_sum(ll);
+        nseq += (uint32_t)__popcll(sel);
+        anchor = (uint32_t)__builtin_amdgcn_readlane((int)end_sel, (int)last);
+        // lanes whose position lies inside a selected match are spent
         const uint32_t spent = stride == 1 ? anchor - pos : (anchor - pos + stride - 1) / stride;
         mask = spent >= (uint32_t)kWave ? 0ull : mask & (~0ull << spent);
       } while (mask && nseq < kMaxSeq);
       pos = anchor;
     }
   }
+  // sequences were stored by the lanes that found them; later passes read
+  // them from every lane
+  wave_sync();
   // one literal plus one offset-1 match to the end: every byte equals in[0]
   if (nseq == 1 && anchor == len) {
     uint32_t rle = 0;
@@ -1349,9 +1374,15 @@ template<> struct WorkerLds<true>
   alignas(16) uint8_t in[kZstdStageBytes + 64];
 };
 
+// Work item = one block of `block` bytes. Output segments (one per `sub`
+// consecutive blocks) sit seg_pitch apart in a chunk's slot; block h of a
+// segment writes at h * sub_off in it (sub = 2: two half-segment blocks by
+// two wavefronts, joined afterwards by zstd_join_k). sizes[chunk * nseg * sub
+// + block index] receives each block's bytes (frame header included).
 template<bool Staged>
 __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
                                              uint32_t chunk_bytes, uint32_t block, uint32_t nseg,
+                                             uint32_t sub, uint32_t sub_off, uint64_t seg_pitch,
                                              uint64_t total_blocks, uint8_t* __restrict__ dst,
                                              uint64_t chunk_dst_stride,
                                              uint32_t* __restrict__ seg_sizes, uint8_t* scratch,
@@ -1376,11 +1407,12 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
     }
     if (item >= total_blocks)
       break;
-    const uint64_t chunk = item / nseg;
-    const uint32_t b = (uint32_t)(item % nseg);
+    const uint32_t per_chunk = nseg * sub;
+    const uint64_t chunk = item / per_chunk;
+    const uint32_t b = (uint32_t)(item % per_chunk); // block index in the chunk
     const uint8_t* in = src + chunk * src_stride + (uint64_t)b * block;
     const uint32_t len = (chunk_bytes - b * block) < block ? chunk_bytes - b * block : block;
-    uint8_t* out = dst + chunk * chunk_dst_stride + (uint64_t)b * (block + kSegSlack);
+    uint8_t* out = dst + chunk * chunk_dst_stride + (uint64_t)(b / sub) * seg_pitch + (b % sub) * sub_off;
     const ChunkTable* tab = tables ? tables + chunk : nullptr;
     uint32_t hdr = 0, skip = 0;
     if (b == 0) {
@@ -1403,10 +1435,10 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
       __syncthreads();
       blk = W.in;
     }
-    const uint32_t sz = compress_block(L, blk + skip, len - skip, out + hdr, b + 1 == nseg, lits, seqs,
+    const uint32_t sz = compress_block(L, blk + skip, len - skip, out + hdr, b + 1 == per_chunk, lits, seqs,
                                        seqout, aux, *pre, tab);
     if (lane == 0)
-      seg_sizes[chunk * nseg + b] = hdr + sz;
+      seg_sizes[chunk * per_chunk + b] = hdr + sz;
     __syncthreads();
     if (!queue)
       break;
@@ -1542,14 +1574,90 @@ hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t 
   const dim3 grid(queue_counter ? workers : (uint32_t)total);
   // staged workers need 16-byte aligned block starts
   const bool staged = block <= kZstdStageBytes && (src_stride % 16) == 0 && ((uintptr_t)src % 16) == 0;
+  const uint64_t pitch = block + kSegSlack;
   if (staged)
     hipLaunchKernelGGL(zstd_k<true>, grid, dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
-                       block, nseg, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
+                       block, nseg, 1u, 0u, pitch, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
                        queue_counter, pre, tabs);
   else
     hipLaunchKernelGGL(zstd_k<false>, grid, dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
-                       block, nseg, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
+                       block, nseg, 1u, 0u, pitch, total, dst, chunk_dst_stride, seg_sizes, (uint8_t*)scratch,
                        queue_counter, pre, tabs);
+  return hipGetLastError();
+}
+
+namespace {
+// One wavefront per segment of a two-block-per-segment launch: the second
+// block's bytes move down behind the first's (the ranges may overlap; the
+// destination lies below the source and every round reads its 1 KiB before
+// writing it, so no byte is overwritten before it is read) and the segment's
+// size becomes their sum.
+__global__ __launch_bounds__(64) void zstd_join_k(uint8_t* __restrict__ dst, uint64_t chunk_dst_stride,
+                                                  uint32_t nseg, uint64_t seg_pitch, uint32_t sub_off,
+                                                  const uint32_t* __restrict__ sub_sizes,
+                                                  uint32_t* __restrict__ seg_sizes, uint64_t n_segments)
+{
+  const uint64_t g = blockIdx.x;
+  if (g >= n_segments)
+    return;
+  const uint64_t chunk = g / nseg;
+  const uint32_t b = (uint32_t)(g % nseg);
+  const uint32_t a_sz = sub_sizes[2 * g], b_sz = sub_sizes[2 * g + 1];
+  uint8_t* seg = dst + chunk * chunk_dst_stride + (uint64_t)b * seg_pitch;
+  const uint8_t* from = seg + sub_off;
+  uint8_t* to = seg + a_sz;
+  const int lane = lane_id();
+  for (uint32_t o = 0; o < b_sz; o += 16u * kWave) {
+    uint8_t v[16];
+    const uint32_t q = o + 16u * (uint32_t)lane;
+    const uint32_t k = q < b_sz ? min(16u, b_sz - q) : 0u;
+    for (uint32_t i = 0; i < k; ++i)
+      v[i] = from[q + i];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = 0; i < k; ++i)
+      to[q + i] = v[i];
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0)
+    seg_sizes[g] = a_sz + b_sz;
+}
+} // namespace
+
+bool zstd_halves_ok(uint64_t chunk_bytes, uint32_t seg_block)
+{
+  return seg_block >= 2 * kZstdBlockMin && seg_block <= 64 * 1024 && chunk_bytes % seg_block == 0 &&
+         seg_block / 2 <= kZstdStageBytes;
+}
+
+hipError_t launch_zstd_halves(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                              uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
+                              uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
+                              uint32_t* queue_counter, hipStream_t stream)
+{
+  if (n == 0)
+    return hipSuccess;
+  const zcore::Predefined* pre = predefined_on_current_device();
+  if (!pre)
+    return hipErrorNotInitialized;
+  if (!zstd_halves_ok(chunk_bytes, seg_block) || !queue_counter || (src_stride % 16) ||
+      ((uintptr_t)src % 16))
+    return hipErrorInvalidValue;
+  const uint32_t block = seg_block / 2;
+  const uint32_t nseg = zstd_segments(chunk_bytes, seg_block);
+  const uint64_t pitch = zstd_segment_stride(seg_block);
+  // the first block of a segment writes at most its frame header, block
+  // header and raw bytes below sub_off; the second fits above it
+  const uint32_t sub_off = (uint32_t)(pitch / 2) & ~15u;
+  const uint64_t total = n * nseg * 2;
+  hipLaunchKernelGGL(zstd_k<true>, dim3(workers), dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
+                     block, nseg, 2u, sub_off, pitch, total, dst, chunk_dst_stride, sub_sizes,
+                     (uint8_t*)scratch, queue_counter, pre, (const ChunkTable*)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  const uint64_t segs = n * nseg;
+  hipLaunchKernelGGL(zstd_join_k, dim3((uint32_t)segs), dim3(64), 0, stream, dst, chunk_dst_stride, nseg, pitch,
+                     sub_off, sub_sizes, seg_sizes, segs);
   return hipGetLastError();
 }
 

@@ -42,6 +42,10 @@ struct tile_stream_gpu_options
   uint32_t lz4_segment_bytes;      /* LZ4 input parsed by one wavefront: a power of two
                                       >= 1024 below the chunk size, or >= the chunk size for
                                       one wavefront per chunk; 0 = autotuned */
+  uint32_t pass_timing;            /* event timing of each scatter and LOD launch for the
+                                      stream metrics: 0/1 = on, 2 = off (those launches then
+                                      run back to back with no event markers between them;
+                                      the compress and copy metrics are kept) */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);
