@@ -465,6 +465,9 @@ int deliver_one(tile_stream_gpu* s, BatchedState* b, bool block)
           break;
         if (q != hipErrorNotReady)
           BCHECK(q);
+        // a stream query's "not ready" stays behind as the thread's last
+        // error, which the next launch check would take for its own
+        (void)hipGetLastError();
         if (!block)
           return 2;
         lk.unlock();

@@ -429,7 +429,10 @@ int dispatch_h2d(tile_stream_gpu* s)
   if (bytes == 0)
     return 0;
   const int i = s->stage_cur;
-  HIPCHECK(hipMemcpyAsync(s->d_stage[i], s->h_stage[i], bytes, hipMemcpyHostToDevice, s->st_h2d));
+  const size_t sent = s->stage_sent;
+  if (sent < bytes)
+    HIPCHECK(hipMemcpyAsync(s->d_stage[i] + sent, s->h_stage[i] + sent, bytes - sent, hipMemcpyHostToDevice,
+                            s->st_h2d));
   HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
   return 0;
 }
@@ -472,6 +475,7 @@ int dispatch_rest(tile_stream_gpu* s)
     HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
   s->cursor += n;
   s->stage_fill = 0;
+  s->stage_sent = 0;
   s->stage_cur = (s->stage_cur + 1) % tile_stream_gpu::kStageSlots;
   s->metrics.h2d.input_bytes += (double)bytes;
   s->metrics.h2d.count++;
@@ -611,6 +615,10 @@ void prefetch_drop(tile_stream_gpu* s)
   }
 }
 
+// Staging copies that nothing ran ahead of go in sub-ranges of this size,
+// each H2D-copied as soon as it is staged.
+constexpr size_t kStageSub = 2u << 20;
+
 // Starts the host copy of the piece the append loop takes next: the rest of
 // the current epoch segment (`seg_left` bytes at `next`), else the start of
 // the next segment. Called between a piece's H2D copy and its launches, when
@@ -715,7 +723,23 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
           prefetch_drop(s);
           if (s->stage_fill == 0 && hipEventSynchronize(s->ev_stage_free[i]) != hipSuccess)
             return writer_error_at(src, end);
-          s->copy_pool->copy(s->h_stage[i] + s->stage_fill, src + written, payload);
+          if (payload >= 2 * kStageSub && s->stage_sent == s->stage_fill) {
+            // nothing copied ahead: the H2D copy starts after the first
+            // sub-range instead of after the whole piece
+            for (size_t o = 0; o < payload; o += kStageSub) {
+              const size_t m = std::min(kStageSub, payload - o);
+              const size_t at = s->stage_fill + o;
+              s->copy_pool->copy(s->h_stage[i] + at, src + written + o, m);
+              if (hipMemcpyAsync(s->d_stage[i] + at, s->h_stage[i] + at, m, hipMemcpyHostToDevice,
+                                 s->st_h2d) != hipSuccess) {
+                set_error("append: H2D copy failed");
+                return writer_error_at(src, end);
+              }
+            }
+            s->stage_sent = s->stage_fill + payload;
+          } else {
+            s->copy_pool->copy(s->h_stage[i] + s->stage_fill, src + written, payload);
+          }
         }
         add_metric(s->metrics.memcpy,
                    std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count(),
@@ -1137,6 +1161,7 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   s->at_capacity = 0;
   std::fill(s->slot_compressed.begin(), s->slot_compressed.end(), (uint8_t)0);
   s->stage_fill = 0;
+  s->stage_sent = 0;
   s->epoch_pending = false;
   s->lod_src = nullptr;
   s->lod_src_free = nullptr;

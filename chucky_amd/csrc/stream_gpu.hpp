@@ -43,6 +43,7 @@ struct tile_stream_gpu
   uint8_t* d_stage[kStageSlots] = {};
   hipEvent_t ev_h2d[kStageSlots] = {}, ev_stage_free[kStageSlots] = {};
   size_t stage_fill = 0;
+  size_t stage_sent = 0; // bytes of the current slot whose H2D copy is already queued
   int stage_cur = 0;
   // host appends: a finished epoch's launches are queued right after the
   // next piece's H2D copy (so the copy engine moves the next epoch while the
