@@ -287,6 +287,12 @@ def leg(name, steps, warmup, host_input, frames=512, options=None):
     rl = compress_roofline(runs["batched"], name)
     if rl:
         out["compressor"] = {k: rl[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms")}
+    lod = lod_rates(runs["batched"])
+    if lod:
+        out["lod"] = lod
+    t = transpose_rate(runs["batched"])
+    if t:
+        out["transpose"] = t
     data.free()
     return out
 
@@ -411,8 +417,9 @@ def main():
                     help="zstd Huffman tables: 0 = autotuned, 1 = one per chunk, 2 = one per block")
     ap.add_argument("--batched-launch", type=int, default=0,
                     help="batched pipeline per-epoch work: 0 = autotuned, 1 = captured graphs, 2 = direct")
-    ap.add_argument("--pass-timing", type=int, default=0,
-                    help="event timing of each scatter / LOD launch: 0/1 = on, 2 = off")
+    ap.add_argument("--pass-timing", type=int, default=2,
+                    help="event timing of each scatter / LOD launch in the timed steps: 1 = on, 2 = off "
+                         "(the default: the per-pass rates then come from a separate HBM-input leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device copy of BYTES (the vector scatter kernel over an "
                          "identity layout), the reference for counter-based HBM traffic")
@@ -478,7 +485,8 @@ def main():
         "data": "synthetic, " + ("appended from host memory (pinned staging + H2D inside the timed region)"
                                  if host_input else "resident in HBM before the timed region"),
         "config": {"workload": desc, "pipeline": lead, "input_GB": round(data.nbytes / 1e9, 4),
-                   "input_from": "host memory" if host_input else "HBM"},
+                   "input_from": "host memory" if host_input else "HBM",
+                   "per_pass_event_timing": args.pass_timing != 2},
         "GiB_per_s": head["GiBps"],
         "compression_ratio": head["ratio"],
         "p50_epoch_latency_ms": head["p50_epoch_latency_ms"],
@@ -501,7 +509,13 @@ def main():
     data.free()
     if not args.no_extras and args.workload == "multiscale" and args.pipeline == "both":
         extras = {}
-        extras["multiscale_hbm_input"] = leg("multiscale", args.steps, args.warmup, False)
+        # input resident in HBM, each scatter / LOD launch timed by events:
+        # the per-pass HBM rates of the headline workload
+        extras["multiscale_hbm_input"] = leg("multiscale", args.steps, args.warmup, False,
+                                             options={"pass_timing": 1})
+        if "lod" in extras["multiscale_hbm_input"]:
+            line["lod"] = dict(extras["multiscale_hbm_input"]["lod"],
+                               note="from the HBM-input leg (extras.multiscale_hbm_input), every pass timed")
         extras["multiscale6_host_input"] = leg("multiscale6", max(3, args.steps // 4), 1, True)
         extras["volume_lz4_host_input"] = leg("volume", max(3, args.steps // 4), 1, True)
         extras["sensor_host_input"] = leg("sensor", 2, 1, True, frames=512)
