@@ -608,63 +608,77 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
     if (lane == 0)
       hsz = (uint32_t)zcore::fse_write_ncount(ws.fse_buf, ws.wnorm, maxw, 6);
     hsz = wave_bcast(hsz, 0);
-    // the weights' two-state stream, backward, even positions on state `se`:
-    // the state chains run in scalar registers and record each step's
-    // (bits, count) in LDS; the wave then packs the records in parallel
+    // the weights' two-state stream, backward, even positions on state `se`.
+    // Every weight's symbol transform is gathered first (one lane-indexed
+    // shuffle per register), so a step of the state chains -- which run in
+    // scalar registers -- reads one lane for its transform and one for its
+    // next state; the step's (bits, count) record goes into the lane of the
+    // weight's own index with a lane write (no LDS store, no exec mask).
     auto rl = [](uint32_t v, uint32_t i) -> uint32_t {
       return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
     };
-    auto weight = [&](int i) -> uint32_t {
-      const uint32_t v = (i >> 6) == 0 ? wt[0] : (i >> 6) == 1 ? wt[1] : (i >> 6) == 2 ? wt[2] : wt[3];
+    uint32_t ew[4], rec[4] = { 0u, 0u, 0u, 0u };
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      ew[k] = (uint32_t)__shfl((int)symv, (int)(wt[k] & 63u), kWave);
+    auto tr = [&](int i) -> uint32_t {
+      const uint32_t v = (i >> 6) == 0 ? ew[0] : (i >> 6) == 1 ? ew[1] : (i >> 6) == 2 ? ew[2] : ew[3];
       return rl(v, (uint32_t)(i & 63));
     };
-    auto init = [&](uint32_t sym) -> uint32_t {
-      const uint32_t e = rl(symv, sym);
+    auto init = [&](uint32_t e) -> uint32_t {
       const uint32_t d = e & 0xFFFFFu;
       const int f = (int)(e >> 20) - 128;
       const uint32_t nb = (d + (1u << 15)) >> 16;
       const uint32_t v = (nb << 16) - d;
       return rl(stv, (uint32_t)((int)(v >> nb) + f));
     };
-    uint16_t* rec = (uint16_t*)ws.w; // step j: (bits << 3) | count
     int i = nw - 1;
     uint32_t se, so;
     if (i & 1) {
-      so = init(weight(i));
-      se = init(weight(i - 1));
+      so = init(tr(i));
+      se = init(tr(i - 1));
     } else {
-      se = init(weight(i));
-      so = init(weight(i - 1));
+      se = init(tr(i));
+      so = init(tr(i - 1));
     }
     i -= 2;
-    uint32_t j = 0;
+    const int top = i; // records at weight indices top .. 0, in stream order
 #pragma unroll
     for (int k = 3; k >= 0; --k) {
-      const uint32_t wk = wt[k];
-      for (; i >= 64 * k; --i, ++j) {
-        const uint32_t sym = rl(wk, (uint32_t)(i & 63));
+      const uint32_t ek = ew[k];
+      uint32_t rk = rec[k];
+      for (; i >= 64 * k; --i) {
         const bool odd = i & 1;
         const uint32_t st = odd ? so : se;
-        const uint32_t e = rl(symv, sym);
+        const uint32_t e = rl(ek, (uint32_t)(i & 63));
         const uint32_t d = e & 0xFFFFFu;
         const int f = (int)(e >> 20) - 128;
         const uint32_t nb = (st + d) >> 16;
-        if (lane == 0)
-          rec[j] = (uint16_t)(((st & ((1u << nb) - 1u)) << 3) | nb);
+        const uint32_t rv = ((st & ((1u << nb) - 1u)) << 3) | nb, rl_i = (uint32_t)(i & 63);
+        // (one scalar source per instruction on this target: the lane goes in m0)
+        asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(rk) : "s"(rv), "{m0}"(rl_i));
         const uint32_t ns = rl(stv, (uint32_t)((int)(st >> nb) + f));
         so = odd ? ns : so;
         se = odd ? se : ns;
       }
+      rec[k] = rk;
     }
-    const uint32_t nsteps = j;
-    wave_sync();
-    // pack: lane l takes steps 4l .. 4l+3 (nsteps <= 253)
+    const uint32_t nsteps = (uint32_t)(top + 1);
+    // pack: lane l takes steps 4l .. 4l+3 (nsteps <= 253), step j being the
+    // record of weight index top - j
     uint32_t bits = 0, nb = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint32_t s = 4u * (uint32_t)lane + (uint32_t)q;
-      if (s < nsteps) {
-        const uint32_t r = rec[s];
+      const uint32_t sj = 4u * (uint32_t)lane + (uint32_t)q;
+      const int wi = top - (int)sj;
+      const int src = wi & 63;
+      uint32_t r = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (uint32_t)__shfl((int)rec[k], src, kWave);
+        r = (wi >> 6) == k ? v : r;
+      }
+      if (sj < nsteps) {
         bits |= (r >> 3) << nb;
         nb += r & 7;
       }
