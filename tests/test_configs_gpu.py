@@ -36,11 +36,11 @@ pytestmark = [pytest.mark.gpu,
 PIPELINES = {"plain": ca.PIPELINE_PLAIN, "batched": ca.PIPELINE_BATCHED}
 
 
-def _check(cfg, data, pipeline, pieces=1, device_input=False, want=None, what=""):
+def _check(cfg, data, pipeline, pieces=1, device_input=False, want=None, what="", **options):
     g, _, chunk_bytes = geometry(cfg)
-    print(f"[{what}] pipeline={pipeline} device_input={device_input} pieces={pieces}", flush=True)
+    print(f"[{what}] pipeline={pipeline} device_input={device_input} pieces={pieces} {options}", flush=True)
     gs, gmeta, gcur = run_gpu(cfg, data, pieces=pieces, device_input=device_input,
-                              pipeline=PIPELINES[pipeline])
+                              pipeline=PIPELINES[pipeline], **options)
     if want is None:
         want = run_ref(cfg, data, pieces=1)
     rs, rmeta, rcur = want
@@ -72,6 +72,18 @@ def test_multiscale_zstd_256(volumes, pipeline, vol):
     data = volumes[0 if vol == "xor" else 1]
     _check(cfg, data, pipeline, pieces=2, device_input=(pipeline == "batched"),
            what=f"multiscale_zstd/{vol}")
+
+
+@pytest.mark.parametrize("pipeline", sorted(PIPELINES))
+@pytest.mark.parametrize("pieces", [1, 3])
+def test_multiscale_zstd_256_host_untimed(volumes, pipeline, pieces):
+    # the benchmark's form: host memory appends (the first piece of a call
+    # staged and copied in sub-ranges, the next ones copied ahead), no event
+    # markers between the scatter / LOD launches; the batched pipeline
+    # compresses the coarse levels of the epochs that emit them in half blocks
+    cfg = W.multiscale_zstd()
+    _check(cfg, volumes[0], pipeline, pieces=pieces, device_input=False, pass_timing=2,
+           what="multiscale_zstd/host")
 
 
 @pytest.mark.parametrize("pipeline", sorted(PIPELINES))
