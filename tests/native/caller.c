@@ -5,10 +5,14 @@
  *   caller estimate          memory estimate + chunk-size advice (host only)
  *   caller run STORE [zstd|lz4|none]
  *                            writes a t,y,x u16 acquisition of 10 frames of
- *                            72x96 (chunks 4x32x32, shards 1x2x3 chunks, a
+ *                            64x96 (chunks 4x32x32, shards 1x2x3 chunks, a
  *                            partial last epoch) through zarr_fs_sink, from
  *                            host memory in three appends
  *
+ * The frame's inner dims are chunk multiples: the reference reads appended
+ * elements in the chunk-padded (lifted) shape of the inner dims
+ * (src/stream/config.c:97-103), so unpadded frames of other sizes would land
+ * shifted, in the reference and here alike.
  * Pixel (t, y, x) = (t * 7919 + y * 131 + x * 17) mod 65536. Exit status 0 on
  * success; failures print the failing call. */
 #include "stream.gpu.h"
@@ -18,7 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-enum { T = 10, Y = 72, X = 96 };
+enum { T = 10, Y = 64, X = 96 };
 
 static void set_dims(struct dimension* dims)
 {

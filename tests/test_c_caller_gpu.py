@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def expected():
-    t, y, x = np.meshgrid(np.arange(10), np.arange(72), np.arange(96), indexing="ij")
+    t, y, x = np.meshgrid(np.arange(10), np.arange(64), np.arange(96), indexing="ij")
     return ((t * 7919 + y * 131 + x * 17) & 0xFFFF).astype(np.uint16)
 
 
@@ -27,5 +27,5 @@ def test_c_caller_writes_readable_store(tmp_path, codec):
     r = subprocess.run([exe, "run", str(store), codec], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     arr, meta, stats = read_array(str(store / "0"))
-    assert arr.shape == (10, 72, 96) and stats["codec"] == codec
+    assert arr.shape == (10, 64, 96) and stats["codec"] == codec
     assert np.array_equal(arr, expected())
