@@ -25,12 +25,13 @@ Z-order ranks on tiny boxes, and the level-2 fold chain 11v/32 of
 tests/test_lod.c:919-928.
 
 f16 has no CPU path in the reference; the restatement follows its GPU path
-(float accumulator for the mean, half-precision fold and emit). Pinned: every
-level of the order-only reductions (min, max, median, 2nd extremes) and
-levels 0-1 of the mean equal float16 of the f32 reference; fold and emit
-equal it on integer-valued data, where every step is exact. Otherwise the
-f16 mean beyond level 1 and the f16 fold/emit rounding are parity unpinned
-at bit level (numpy rounds a float32 result; the GPU compares within 1 ULP).
+(float accumulator for the mean, half-precision fold and emit;
+src/gpu/lod.cu:71-75, 413-528, 827-941). Pinned: every level of the
+order-only reductions (min, max, median, 2nd extremes) and levels 0-1 of the
+mean equal float16 of the f32 reference; on general data every pyramid level
+and every fold / emit equal an exact-rational model of that arithmetic (sums,
+divides and half adds on fractions with an independent round-to-nearest-even)
+bit for bit.
 """
 from __future__ import annotations
 
