@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <sys/syscall.h>
 #include <unistd.h>
@@ -55,9 +56,28 @@ int gpu_numa_node(int device)
   return node;
 }
 
-} // namespace
 
-hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device)
+// Pinned buffers of finished streams are kept for the next streams of the
+// process (a serving loop creates a stream per request, and pinning GBs of
+// host memory takes a large part of a stream's set-up): a request takes the
+// smallest free buffer of its device between its size and twice that.
+// Contents are not cleared (every user writes a buffer before reading it).
+struct PinnedPool
+{
+  std::mutex mu;
+  std::map<void*, std::pair<size_t, int>> live; // every buffer made here: size, device
+  std::multimap<size_t, std::pair<void*, int>> free_by_size;
+  size_t free_bytes = 0;
+  static constexpr size_t kMaxFree = (size_t)16 << 30;
+};
+
+PinnedPool& pool()
+{
+  static PinnedPool* p = new PinnedPool(); // kept past static destruction
+  return *p;
+}
+
+hipError_t pin_new(void** p, size_t bytes, int device)
 {
   const int node = gpu_numa_node(device);
   if (node < 0 || node >= 64)
@@ -68,6 +88,49 @@ hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device)
   const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocNumaUser);
   (void)syscall(SYS_set_mempolicy, kMpolDefault, nullptr, 0ul);
   return e;
+}
+
+} // namespace
+
+hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device)
+{
+  PinnedPool& P = pool();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto it = P.free_by_size.lower_bound(bytes); it != P.free_by_size.end() && it->first <= 2 * bytes;
+         ++it)
+      if (it->second.second == device) {
+        *p = it->second.first;
+        P.free_bytes -= it->first;
+        P.free_by_size.erase(it);
+        return hipSuccess;
+      }
+  }
+  const hipError_t e = pin_new(p, bytes, device);
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.live[*p] = { bytes, device };
+  }
+  return e;
+}
+
+void host_release(void* p)
+{
+  if (!p)
+    return;
+  PinnedPool& P = pool();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.live.find(p);
+    if (it != P.live.end() && P.free_bytes + it->second.first <= PinnedPool::kMaxFree) {
+      P.free_by_size.emplace(it->second.first, std::make_pair(p, it->second.second));
+      P.free_bytes += it->second.first;
+      return;
+    }
+    if (it != P.live.end())
+      P.live.erase(it);
+  }
+  (void)hipHostFree(p);
 }
 
 } // namespace chucky

@@ -1122,7 +1122,7 @@ int lod_init(tile_stream_gpu* s)
     set_error("lod: an epoch of more than 2^32 LOD elements is not supported");
     return 1;
   }
-  build_lod_luts(p, &s->luts);
+  cached_lod_luts(p, &s->luts);
   const size_t bpe = p.bpe;
   void* lin = nullptr;
   void* lod = nullptr;

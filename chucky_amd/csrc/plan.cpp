@@ -5,10 +5,42 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 namespace chucky {
 
 namespace {
+
+// fn(lo, hi) over [0, n) split into contiguous ranges on up to 16 threads
+// (the calling thread takes the first); small ranges stay on the caller.
+// The per-element LUTs of a large LOD level are millions of Morton ranks.
+template<typename F>
+void parallel_ranges(uint64_t n, F fn)
+{
+  constexpr uint64_t kMinPerThread = 1u << 15;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const uint64_t nt = std::min<uint64_t>({ (uint64_t)std::min(hw, 16u), n / kMinPerThread, 64 });
+  if (nt <= 1) {
+    fn((uint64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  const uint64_t step = (n + nt - 1) / nt;
+  for (uint64_t t = 1; t < nt; ++t) {
+    const uint64_t lo = std::min(n, t * step), hi = std::min(n, lo + step);
+    th.emplace_back([=] { fn(lo, hi); });
+  }
+  fn((uint64_t)0, std::min(n, step));
+  for (auto& x : th)
+    x.join();
+}
 
 uint64_t max_of(int n, const uint64_t* v)
 {
@@ -388,18 +420,21 @@ int build_plan(const tile_stream_configuration* config, size_t codec_alignment,
       const int zp = ceil_log2(max_of(p.lod_ndim, parent));
       const uint64_t n_par = p.levels[l + 1].lod_nelem;
       uint64_t* seg = p.ends.data() + p.segment_begin(l);
-      for (uint64_t j = 0; j < n_par; ++j) {
-        uint64_t c[LOD_MAX_NDIM], nx[LOD_MAX_NDIM];
-        uint64_t r = j;
-        for (int k = 0; k < p.lod_ndim; ++k) {
-          c[k] = r % parent[k];
-          r /= parent[k];
+      const int nd = p.lod_ndim;
+      parallel_ranges(n_par, [&](uint64_t j0, uint64_t j1) {
+        for (uint64_t j = j0; j < j1; ++j) {
+          uint64_t c[LOD_MAX_NDIM], nx[LOD_MAX_NDIM];
+          uint64_t r = j;
+          for (int k = 0; k < nd; ++k) {
+            c[k] = r % parent[k];
+            r /= parent[k];
+          }
+          const uint64_t pos = morton_rank(nd, parent, c, 0);
+          std::copy(c, c + nd, nx);
+          z_increment(nd, zp, nx);
+          seg[pos] = morton_rank(nd, child, nx, 1);
         }
-        const uint64_t pos = morton_rank(p.lod_ndim, parent, c, 0);
-        std::copy(c, c + p.lod_ndim, nx);
-        z_increment(p.lod_ndim, zp, nx);
-        seg[pos] = morton_rank(p.lod_ndim, child, nx, 1);
-      }
+      });
     }
   }
 
@@ -523,33 +558,46 @@ void build_tiles(int ndim, size_t bpe, const uint64_t* shape, const std::vector<
       grid[k] = shape[k] / side;
       ntiles *= grid[k];
     }
-    for (uint64_t ti = 0; ti < ntiles; ++ti) {
-      uint64_t origin[3] = { 0, 0, 0 }, r = ti;
-      for (int k = ndim - 1; k >= 0; --k) {
-        origin[k] = (r % grid[k]) * side;
-        r /= grid[k];
-      }
-      if (!in_one_piece(origin))
-        continue;
-      const uint64_t base = morton_rank(ndim, shape, origin, 0);
-      if (base + cells > n || base > 0xFFFFFFFFull)
-        continue;
-      const uint32_t off0 = lut[base];
-      bool ok = true;
-      for (uint64_t z = 0; z < cells && ok; ++z) {
-        int64_t off = off0;
-        for (int k = 0; k < ndim; ++k) {
-          uint64_t c = 0;
-          for (int j = 0; j < t->side_log; ++j)
-            c |= ((z >> (j * ndim + k)) & 1ull) << j;
-          off += (int64_t)c * stride[k];
+    // each tile is checked on its own (in parallel); accepted tiles are
+    // then listed in tile order
+    const int side_log = t->side_log;
+    std::vector<uint64_t> tile_base(ntiles, ~0ull);
+    parallel_ranges(ntiles * cells, [&](uint64_t c0, uint64_t c1) {
+      for (uint64_t ti = c0 / cells; ti < ntiles && ti * cells < c1; ++ti) {
+        if (ti * cells < c0)
+          continue; // a tile belongs to the range holding its first cell
+        uint64_t origin[3] = { 0, 0, 0 }, r = ti;
+        for (int k = ndim - 1; k >= 0; --k) {
+          origin[k] = (r % grid[k]) * side;
+          r /= grid[k];
         }
-        ok = (int64_t)lut[base + z] == off;
+        if (!in_one_piece(origin))
+          continue;
+        const uint64_t base = morton_rank(ndim, shape, origin, 0);
+        if (base + cells > n || base > 0xFFFFFFFFull)
+          continue;
+        const uint32_t off0 = lut[base];
+        bool ok = true;
+        for (uint64_t z = 0; z < cells && ok; ++z) {
+          int64_t off = off0;
+          for (int k = 0; k < ndim; ++k) {
+            uint64_t c = 0;
+            for (int j = 0; j < side_log; ++j)
+              c |= ((z >> (j * ndim + k)) & 1ull) << j;
+            off += (int64_t)c * stride[k];
+          }
+          ok = (int64_t)lut[base + z] == off;
+        }
+        if (ok)
+          tile_base[ti] = base;
       }
-      if (!ok)
+    });
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+      const uint64_t base = tile_base[ti];
+      if (base == ~0ull)
         continue;
       t->tiles.push_back((uint32_t)base);
-      t->tiles.push_back(off0);
+      t->tiles.push_back(lut[base]);
       std::fill(covered.begin() + (int64_t)base, covered.begin() + (int64_t)(base + cells), (uint8_t)1);
     }
   }
@@ -596,15 +644,18 @@ void build_lod_luts(const Plan& p, LodLuts* out)
     lshape[k] = p.lod_shape(0, k);
   const uint64_t n0 = p.levels[0].lod_nelem;
   out->gather_lut.assign(n0, 0);
-  for (uint64_t g = 0; g < n0; ++g) {
-    uint64_t c[LOD_MAX_NDIM], r = g, off = 0;
-    for (int k = p.lod_ndim - 1; k >= 0; --k) {
-      c[k] = r % lshape[k];
-      r /= lshape[k];
-      off += c[k] * fstride[p.lod_to_dim[k]];
+  uint32_t* glut = out->gather_lut.data();
+  parallel_ranges(n0, [&](uint64_t g0, uint64_t g1) {
+    for (uint64_t g = g0; g < g1; ++g) {
+      uint64_t c[LOD_MAX_NDIM], r = g, off = 0;
+      for (int k = p.lod_ndim - 1; k >= 0; --k) {
+        c[k] = r % lshape[k];
+        r /= lshape[k];
+        off += c[k] * fstride[p.lod_to_dim[k]];
+      }
+      glut[morton_rank(p.lod_ndim, lshape, c, 0)] = (uint32_t)off;
     }
-    out->gather_lut[morton_rank(p.lod_ndim, lshape, c, 0)] = (uint32_t)off;
-  }
+  });
   out->gather_fixed.assign(p.fixed_count, 0);
   for (uint64_t b = 0; b < p.fixed_count; ++b) {
     uint64_t r = b, off = 0;
@@ -625,19 +676,22 @@ void build_lod_luts(const Plan& p, LodLuts* out)
     const uint64_t n = p.levels[lv].lod_nelem;
     auto& lut = out->chunk_lut[lv];
     lut.assign(n, 0);
-    for (uint64_t g = 0; g < n; ++g) {
-      uint64_t c[LOD_MAX_NDIM], r = g;
-      int64_t off = 0;
-      for (int k = p.lod_ndim - 1; k >= 0; --k) {
-        c[k] = r % ls[k];
-        r /= ls[k];
-        const int d = p.lod_to_dim[k];
-        const uint64_t cs = L.lifted_shape[2 * d + 1];
-        off += (int64_t)(c[k] / cs) * L.lifted_strides[2 * d] +
-               (int64_t)(c[k] % cs) * L.lifted_strides[2 * d + 1];
+    uint32_t* clut = lut.data();
+    parallel_ranges(n, [&](uint64_t g0, uint64_t g1) {
+      for (uint64_t g = g0; g < g1; ++g) {
+        uint64_t c[LOD_MAX_NDIM], r = g;
+        int64_t off = 0;
+        for (int k = p.lod_ndim - 1; k >= 0; --k) {
+          c[k] = r % ls[k];
+          r /= ls[k];
+          const int d = p.lod_to_dim[k];
+          const uint64_t cs = L.lifted_shape[2 * d + 1];
+          off += (int64_t)(c[k] / cs) * L.lifted_strides[2 * d] +
+                 (int64_t)(c[k] % cs) * L.lifted_strides[2 * d + 1];
+        }
+        clut[morton_rank(p.lod_ndim, ls, c, 0)] = (uint32_t)off;
       }
-      lut[morton_rank(p.lod_ndim, ls, c, 0)] = (uint32_t)off;
-    }
+    });
     auto& fx = out->chunk_fixed[lv];
     fx.assign(p.fixed_count, 0);
     for (uint64_t b = 0; b < p.fixed_count; ++b) {
@@ -692,6 +746,66 @@ void build_lod_luts(const Plan& p, LodLuts* out)
       build_tiles(0, p.bpe, lshape, out->chunk_lut[lv], nullptr, out->chunk_fixed[lv],
                   [](const uint64_t*) { return true; }, &out->chunk_tiles[lv]);
   }
+}
+
+// Tile lists, edge lists and batch offsets of every geometry seen, keyed by
+// the plan values they follow from (a serving process creates many streams
+// of a few geometries; a 2048x2048 level costs ~10^7 Morton ranks to build).
+// The per-cell rank LUTs are not kept.
+void cached_lod_luts(const Plan& p, LodLuts* out)
+{
+  std::string key;
+  auto put = [&](uint64_t v) { key.append((const char*)&v, sizeof v); };
+  put((uint64_t)p.rank);
+  put(p.bpe);
+  put((uint64_t)p.nlod);
+  put((uint64_t)p.lod_ndim);
+  put((uint64_t)p.fixed_ndim);
+  put(p.fixed_count);
+  for (int k = 0; k < HALF_MAX_RANK; ++k) {
+    put((uint64_t)p.lod_to_dim[k]);
+    put((uint64_t)p.fixed_to_dim[k]);
+    put(p.fixed_shape[k]);
+  }
+  for (int lv = 0; lv < p.nlod; ++lv) {
+    const LevelInfo& L = p.levels[lv];
+    put(L.lod_nelem);
+    put(L.chunk_offset);
+    put(L.layout.chunk_stride);
+    for (int d = 0; d < p.rank; ++d)
+      put(L.shape[d]);
+    for (int d = 0; d < 2 * p.rank; ++d) {
+      put(L.layout.lifted_shape[d]);
+      put((uint64_t)L.layout.lifted_strides[d]);
+    }
+  }
+  static std::mutex mu;
+  static std::map<std::string, std::shared_ptr<const LodLuts>> cache;
+  static std::deque<std::string> order;
+  std::shared_ptr<const LodLuts> hit;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end())
+      hit = it->second;
+  }
+  if (!hit) {
+    auto built = std::make_shared<LodLuts>();
+    build_lod_luts(p, built.get());
+    built->gather_lut = {};
+    built->chunk_lut = {};
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cache.count(key)) {
+      cache[key] = built;
+      order.push_back(key);
+      while (order.size() > 32) {
+        cache.erase(order.front());
+        order.pop_front();
+      }
+    }
+    hit = built;
+  }
+  *out = *hit;
 }
 
 } // namespace chucky

@@ -168,6 +168,9 @@ struct LodLuts
   std::vector<LodTiles> chunk_tiles;  // level Morton order -> chunk pool, per level
 };
 void build_lod_luts(const Plan& p, LodLuts* out);
+// The tiled forms and batch offsets of build_lod_luts (rank LUTs left
+// empty), from a process-wide cache of the geometries built so far.
+void cached_lod_luts(const Plan& p, LodLuts* out);
 
 // Aggregate LUTs for one level and one batch: entry a*M+j maps the j-th chunk
 // of active epoch a to its pool chunk (gather) and to its slot in the

@@ -821,7 +821,7 @@ void free_stream(tile_stream_gpu* s)
   for (void* p : s->allocations)
     (void)hipFree(p);
   for (void* p : s->host_allocations)
-    (void)hipHostFree(p);
+    host_release(p);
   if (s->st_comp)
     (void)hipStreamDestroy(s->st_comp);
   if (s->st_h2d)

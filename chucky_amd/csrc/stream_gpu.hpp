@@ -173,6 +173,9 @@ void timing_add(tile_stream_gpu* s, hipEvent_t a, hipEvent_t b, stream_metric st
 uint32_t compressor_slots();
 // hipHostMalloc on the NUMA node of `device` (default policy if unknown).
 hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
+// Returns a buffer of host_alloc_near_gpu to the process's pinned pool (or
+// frees it when the pool is full).
+void host_release(void* p);
 // Codec launch shapes from the recorded sweep tables (autotune.cpp), by
 // multiscale or not, dtype size, chunk bytes and chunks per compress launch.
 uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);
