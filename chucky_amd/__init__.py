@@ -129,7 +129,8 @@ class StreamOptions(C.Structure):
                 ("lz4_hash_log", C.c_uint32), ("zstd_block_bytes", C.c_uint32),
                 ("host_gather_workgroups", C.c_uint32), ("zstd_chunk_tables", C.c_uint32),
                 ("batched_d2h", C.c_uint32), ("batched_launch", C.c_uint32),
-                ("lz4_segment_bytes", C.c_uint32), ("pass_timing", C.c_uint32)]
+                ("lz4_segment_bytes", C.c_uint32), ("pass_timing", C.c_uint32),
+                ("h2d_copy", C.c_uint32)]
 
 
 def _sig(name, res, *args):

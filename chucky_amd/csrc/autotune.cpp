@@ -167,6 +167,14 @@ uint32_t tune_host_gather_workgroups()
   return (uint32_t)(cus / 4 > 0 ? cus / 4 : 1);
 }
 
+uint32_t tune_h2d_copy_workgroups()
+{
+  // PCIe-bound loads from pinned memory, measured on MI355X for 2 and 8 MiB
+  // pieces: 32 workgroups of 256 lanes reach 50 and 55 GB/s; 16 fall short
+  // on 8 MiB (48 GB/s) and 512 or more lose to contention (39-52 GB/s)
+  return 32;
+}
+
 } // namespace chucky
 
 extern "C" int chucky_tune_codec(enum compression_codec codec, int multiscale, uint32_t bpe,

@@ -23,6 +23,15 @@ struct LiftedSpace
 hipError_t launch_transpose(void* dst, const void* src, uint64_t n, int bpe, uint64_t i_offset,
                             const LiftedSpace& space, hipStream_t stream);
 
+// ---- host-to-device copy by a kernel ----
+
+// True when launch_h2d_copy can copy between these addresses (they agree
+// modulo 16).
+bool h2d_copy_ok(const void* dst, const void* src);
+// dst[0, n) = src[0, n), src a device address of pinned host memory, read by
+// at most `groups` workgroups of 256 lanes.
+hipError_t launch_h2d_copy(void* dst, const void* src, uint64_t n, uint32_t groups, hipStream_t stream);
+
 // ---- aggregate ----
 
 // Sizes of the compressed segments. A chunk is nseg segments; segment s of

@@ -423,6 +423,13 @@ int settle_epoch(tile_stream_gpu* s)
 // the chunk scatter, or for multiscale streams a device copy into the epoch
 // buffer -- unless the piece is a whole epoch, which the LOD gather then
 // reads straight from the staging slot.
+hipError_t h2d_copy(tile_stream_gpu* s, int i, size_t at, size_t n)
+{
+  if (s->h2d_groups)
+    return launch_h2d_copy(s->d_stage[i] + at, s->h_stage_dev[i] + at, n, s->h2d_groups, s->st_h2d);
+  return hipMemcpyAsync(s->d_stage[i] + at, s->h_stage[i] + at, n, hipMemcpyHostToDevice, s->st_h2d);
+}
+
 int dispatch_h2d(tile_stream_gpu* s)
 {
   const size_t bytes = s->stage_fill;
@@ -431,8 +438,7 @@ int dispatch_h2d(tile_stream_gpu* s)
   const int i = s->stage_cur;
   const size_t sent = s->stage_sent;
   if (sent < bytes)
-    HIPCHECK(hipMemcpyAsync(s->d_stage[i] + sent, s->h_stage[i] + sent, bytes - sent, hipMemcpyHostToDevice,
-                            s->st_h2d));
+    HIPCHECK(h2d_copy(s, i, sent, bytes - sent));
   HIPCHECK(hipEventRecord(s->ev_h2d[i], s->st_h2d));
   return 0;
 }
@@ -730,8 +736,7 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
               const size_t m = std::min(kStageSub, payload - o);
               const size_t at = s->stage_fill + o;
               s->copy_pool->copy(s->h_stage[i] + at, src + written + o, m);
-              if (hipMemcpyAsync(s->d_stage[i] + at, s->h_stage[i] + at, m, hipMemcpyHostToDevice,
-                                 s->st_h2d) != hipSuccess) {
+              if (h2d_copy(s, i, at, m) != hipSuccess) {
                 set_error("append: H2D copy failed");
                 return writer_error_at(src, end);
               }
@@ -895,6 +900,28 @@ int create_impl(tile_stream_gpu* s, const tile_stream_configuration* config, sha
     HIPCHECK(hipEventCreateWithFlags(&s->ev_h2d[i], hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&s->ev_stage_free[i], hipEventDisableTiming));
     HIPCHECK(hipEventRecord(s->ev_stage_free[i], s->st_comp));
+  }
+  // The H2D copy of staged pieces: the DMA engines, or on request a kernel
+  // reading the pinned buffers through their device addresses (launches that
+  // wait for it are then queued on the device instead of holding the host
+  // thread until a DMA copy ends; in the pipelines it runs slower, sharing
+  // the CUs and the fabric with the compressors)
+  if (s->opts.h2d_copy > 2) {
+    set_error("create: h2d_copy must be 0, 1 or 2");
+    return 1;
+  }
+  if (s->opts.h2d_copy == 1) {
+    for (int i = 0; i < tile_stream_gpu::kStageSlots; ++i) {
+      void* dp = nullptr;
+      if (hipHostGetDevicePointer(&dp, s->h_stage[i], 0) != hipSuccess || !dp ||
+          !h2d_copy_ok(s->d_stage[i], dp)) {
+        (void)hipGetLastError();
+        set_error("create: h2d_copy = 1 but the pinned staging buffers have no device address");
+        return 1;
+      }
+      s->h_stage_dev[i] = (uint8_t*)dp;
+    }
+    s->h2d_groups = tune_h2d_copy_workgroups();
   }
 
   // pool (zeroed once; padding positions stay zero for the stream's life)

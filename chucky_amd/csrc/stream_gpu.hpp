@@ -41,6 +41,10 @@ struct tile_stream_gpu
   size_t stage_cap = 0;
   uint8_t* h_stage[kStageSlots] = {};
   uint8_t* d_stage[kStageSlots] = {};
+  // device addresses of h_stage, read by the H2D copy kernel (option
+  // h2d_copy = 1); h2d_groups is its grid (0: the DMA engines copy instead)
+  uint8_t* h_stage_dev[kStageSlots] = {};
+  uint32_t h2d_groups = 0;
   hipEvent_t ev_h2d[kStageSlots] = {}, ev_stage_free[kStageSlots] = {};
   size_t stage_fill = 0;
   size_t stage_sent = 0; // bytes of the current slot whose H2D copy is already queued
@@ -187,6 +191,8 @@ struct Lz4Choice
 };
 Lz4Choice tune_lz4(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);
 uint32_t tune_host_gather_workgroups();
+// grid of the H2D copy kernel of staged pieces
+uint32_t tune_h2d_copy_workgroups();
 // LZ4 wavefronts resident at once on the device (LDS-bound: hash table and
 // staged segment)
 uint64_t lz4_wave_slots(int hash_log, uint32_t seg_bytes);

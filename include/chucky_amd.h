@@ -46,6 +46,9 @@ struct tile_stream_gpu_options
                                       stream metrics: 0/1 = on, 2 = off (those launches then
                                       run back to back with no event markers between them;
                                       the compress and copy metrics are kept) */
+  uint32_t h2d_copy;               /* host appends, staging -> device: 0/2 = the DMA engines
+                                      (hipMemcpyAsync), 1 = a copy kernel reading the pinned
+                                      staging buffer */
 };
 
 void tile_stream_gpu_options_default(struct tile_stream_gpu_options* o);

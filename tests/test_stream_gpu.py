@@ -120,6 +120,27 @@ def test_partial_last_epoch_and_small_buffer():
     _compare(cfg, data, pieces=7)
 
 
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_h2d_copy_kernel(pipeline):
+    # staged pieces moved to the device by the copy kernel (h2d_copy = 1):
+    # u8 pieces through a 4 KiB staging buffer start at every alignment
+    cfg = ca.Config([Dim(20, 8, 2), Dim(16, 8, 1), Dim(24, 8, 3)], dtype="u8",
+                    buffer_capacity_bytes=4096)
+    data = random_typed(20 * 16 * 24, "u8", seed=11)
+    _compare(cfg, data, pieces=7, pipeline=pipeline, h2d_copy=1)
+
+
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_h2d_copy_kernel_epochs(pipeline):
+    # whole 8 MiB epochs of a multiscale stream by the copy kernel, the first
+    # one in 2 MiB sub-ranges as it is staged
+    v = xor_volume((128, 256, 256))
+    cfg = ca.Config([Dim(128, 64, 2, name="z", downsample=True), Dim(256, 64, 4, name="y", downsample=True),
+                     Dim(256, 64, 4, name="x", downsample=True)], dtype="u16", codec="zstd", level=1,
+                    reduce="mean", append_reduce="mean", buffer_capacity_bytes=16 << 20)
+    _compare(cfg, v, pieces=1, pipeline=pipeline, h2d_copy=1)
+
+
 def test_device_input_matches_host_input():
     v = xor_volume((32, 64, 64))
     cfg = ca.Config([Dim(32, 8, 2), Dim(64, 32, 1), Dim(64, 32, 2)], dtype="u16", codec="zstd")
