@@ -82,32 +82,58 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v)
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-// Inclusive prefix sum over the wave.
+// Cross-lane steps by DPP (data-parallel primitives: a VALU operand read
+// from another lane of the same row of 16 or broadcast across rows), a few
+// cycles each, where a shuffle is an LDS-unit permute with its full latency.
+// Source lanes outside the row read as 0 (bound_ctrl), lanes outside
+// row_mask keep their value. The reductions and scans below need every lane
+// of the wave active (they are called from converged code only).
+template<int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_or0(uint32_t v)
+{
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+// Inclusive prefix sum over the wave: Hillis-Steele within each row of 16
+// lanes, then row 0's total into row 1 and row 2's into row 3, then the
+// first half's total into the second half.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-  const int lane = lane_id();
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t u = __shfl_up(v, off, kWave);
-    if (lane >= off)
-      v += u;
-  }
+  v += dpp_or0<kDppRowShr1>(v);
+  v += dpp_or0<kDppRowShr2>(v);
+  v += dpp_or0<kDppRowShr4>(v);
+  v += dpp_or0<kDppRowShr8>(v);
+  v += dpp_or0<kDppRowBcast15, 0xA>(v);
+  v += dpp_or0<kDppRowBcast31, 0xC>(v);
   return v;
 }
 
+// Sum over the wave, as a wave-uniform (scalar) value.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
-  for (int off = kWave / 2; off > 0; off >>= 1)
-    v += __shfl_xor(v, off, kWave);
-  return v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), kWave - 1);
 }
 
+// Maximum over the wave, as a wave-uniform value (the same running scheme
+// with max in place of +; 0 is its identity for unsigned values).
 __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 {
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    const uint32_t u = __shfl_xor(v, off, kWave);
-    v = u > v ? u : v;
-  }
-  return v;
+  uint32_t u;
+  u = dpp_or0<kDppRowShr1>(v);
+  v = u > v ? u : v;
+  u = dpp_or0<kDppRowShr2>(v);
+  v = u > v ? u : v;
+  u = dpp_or0<kDppRowShr4>(v);
+  v = u > v ? u : v;
+  u = dpp_or0<kDppRowShr8>(v);
+  v = u > v ? u : v;
+  u = dpp_or0<kDppRowBcast15, 0xA>(v);
+  v = u > v ? u : v;
+  u = dpp_or0<kDppRowBcast31, 0xC>(v);
+  v = u > v ? u : v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1);
 }
 
 // A 512-entry array spread over the wave's registers: entry i lives in lane

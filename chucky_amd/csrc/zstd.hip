@@ -643,23 +643,34 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
     }
     i -= 2;
     const int top = i; // records at weight indices top .. 0, in stream order
+    // one step of a state chain over weight index `wi`: its (bits, count)
+    // record into lane wi % 64 of rk, the next state returned
+    auto step = [&](uint32_t ek, uint32_t& rk, int wi, uint32_t st) -> uint32_t {
+      const uint32_t e = rl(ek, (uint32_t)(wi & 63));
+      const uint32_t d = e & 0xFFFFFu;
+      const int f = (int)(e >> 20) - 128;
+      const uint32_t nb = (st + d) >> 16;
+      const uint32_t rv = ((st & ((1u << nb) - 1u)) << 3) | nb, rl_i = (uint32_t)(wi & 63);
+      // (one scalar source per instruction on this target: the lane goes in m0)
+      asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(rk) : "s"(rv), "{m0}"(rl_i));
+      return rl(stv, (uint32_t)((int)(st >> nb) + f));
+    };
+    // Odd indices run on `so`, even ones on `se`. After a first even step
+    // (when top is even) the steps go in (odd, even) pairs, whose two chains
+    // are independent and overlap; a pair never straddles a register (the
+    // 64-index boundaries are even).
 #pragma unroll
     for (int k = 3; k >= 0; --k) {
       const uint32_t ek = ew[k];
       uint32_t rk = rec[k];
-      for (; i >= 64 * k; --i) {
-        const bool odd = i & 1;
-        const uint32_t st = odd ? so : se;
-        const uint32_t e = rl(ek, (uint32_t)(i & 63));
-        const uint32_t d = e & 0xFFFFFu;
-        const int f = (int)(e >> 20) - 128;
-        const uint32_t nb = (st + d) >> 16;
-        const uint32_t rv = ((st & ((1u << nb) - 1u)) << 3) | nb, rl_i = (uint32_t)(i & 63);
-        // (one scalar source per instruction on this target: the lane goes in m0)
-        asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(rk) : "s"(rv), "{m0}"(rl_i));
-        const uint32_t ns = rl(stv, (uint32_t)((int)(st >> nb) + f));
-        so = odd ? ns : so;
-        se = odd ? se : ns;
+      if (i >= 64 * k && !(i & 1)) {
+        se = step(ek, rk, i, se);
+        --i;
+      }
+      for (; i >= 64 * k + 1; i -= 2) {
+        const uint32_t no = step(ek, rk, i, so);
+        se = step(ek, rk, i - 1, se);
+        so = no;
       }
       rec[k] = rk;
     }
