@@ -498,6 +498,129 @@ __device__ void huf_build_wave(Lds& L, uint32_t n_in)
   wave_sync();
 }
 
+// Huffman code lengths (at most kHufMaxBits, a complete prefix code) and
+// canonical values for a block's literals, counts in L.hist (at least two
+// used symbols, every count below 2^17), without sorting: each lane holds
+// symbols 64r + lane, r < 4.
+//  1. Shannon lengths l = ceil(log2(M / c)) capped at kHufMaxBits, with
+//     M = N (the literal count) when no symbol falls below the cap, else
+//     M = N + ceil(N / 7): then the uncapped terms of the Kraft sum add up
+//     to at most 7/8 and the (at most 256) capped ones to at most 1/8, so
+//     the code is never over-full.
+//  2. The Kraft sum is filled to exactly 1 greedily, shortening the codes
+//     with the largest count per unit of Kraft space they take (count * 2^l;
+//     shortening a length-l code takes 2^(11-l) units of 2^-11). A round
+//     takes the best code's length class at once: every code of that class
+//     whose ratio is still at least the best of the other classes, as many
+//     as fit, in symbol order. A round always finds a code that fits (the
+//     remaining space is a multiple of the longest code's unit), so the loop
+//     ends with a complete code.
+// Measured against package-merge (optimal length-limited codes) on 8 KiB
+// blocks of imaging data, skewed and uniform synthetic bytes: 0.00-0.4% more
+// literal bits, in 2-45 rounds (about 14 on imaging data). Returns false
+// (no code) only if the fill found no code to shorten, which the argument
+// above rules out.
+__device__ bool huf_build_fast_wave(Lds& L)
+{
+  const uint32_t lane = (uint32_t)lane_id();
+  constexpr uint32_t Lmax = zcore::kHufMaxBits;
+  constexpr uint32_t kFull = 1u << Lmax;
+  uint32_t c[4], l[4];
+  uint32_t nloc = 0, rare = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    c[r] = L.hist[64u * r + lane];
+    nloc += c[r];
+  }
+  const uint32_t N = wave_sum(nloc);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    rare |= (c[r] && (c[r] << Lmax) < N) ? 1u : 0u;
+  const uint32_t M = wave_ballot(rare != 0) ? N + (N + 6) / 7 : N;
+  const uint32_t clz_m = (uint32_t)__builtin_clz(M);
+  uint32_t kloc = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    uint32_t v = 0;
+    if (c[r]) {
+      const uint32_t l0 = (uint32_t)__builtin_clz(c[r]) - clz_m; // c <= M
+      v = (c[r] << l0) >= M ? l0 : l0 + 1;
+      v = v < 1 ? 1 : (v > Lmax ? Lmax : v);
+      kloc += kFull >> v;
+    }
+    l[r] = v;
+  }
+  uint32_t K = wave_sum(kloc);
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  while (K < kFull) {
+    const uint32_t R = kFull - K;
+    uint32_t key[4], kmax = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      key[r] = (l[r] > 1 && (kFull >> l[r]) <= R) ? c[r] << l[r] : 0u; // c < 2^17: below 2^28
+      kmax = key[r] > kmax ? key[r] : kmax;
+    }
+    kmax = wave_max(kmax);
+    if (kmax == 0)
+      return false;
+    // the class of the best ratio (the shortest, when classes tie)
+    uint32_t inv = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      inv = key[r] == kmax && 16u - l[r] > inv ? 16u - l[r] : inv;
+    const uint32_t ls = 16u - wave_max(inv);
+    uint32_t k2 = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      k2 = l[r] != ls && key[r] > k2 ? key[r] : k2;
+    k2 = wave_max(k2);
+    const uint32_t m = R >> (Lmax - ls); // codes of the class that fit
+    uint32_t base = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool e = key[r] != 0 && l[r] == ls && key[r] >= k2;
+      const uint64_t b = wave_ballot(e);
+      if (e && base + (uint32_t)__popcll(b & below) < m)
+        l[r] -= 1;
+      base += (uint32_t)__popcll(b);
+    }
+    K += (base < m ? base : m) << (Lmax - ls);
+  }
+  uint32_t maxd = 0, maxsym = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    L.code.len[64u * r + lane] = (uint8_t)l[r];
+    maxd = l[r] > maxd ? l[r] : maxd;
+    maxsym = c[r] ? 64u * r + lane : maxsym;
+  }
+  maxd = wave_max(maxd);
+  maxsym = wave_max(maxsym);
+  // canonical values: per length, consecutive in symbol order, starting
+  // values from the longest length down
+  uint32_t val[4] = { 0, 0, 0, 0 };
+  uint32_t minv = 0;
+  for (uint32_t d = maxd; d >= 1; --d) {
+    uint32_t rank = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint64_t b = wave_ballot(l[r] == d);
+      if (l[r] == d)
+        val[r] = minv + rank + (uint32_t)__popcll(b & below);
+      rank += (uint32_t)__popcll(b);
+    }
+    minv = (minv + rank) >> 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    L.code.val[64u * r + lane] = (uint16_t)val[r];
+  if (lane == 0) {
+    L.code.max_bits = (int)maxd;
+    L.code.max_sym = (int)maxsym;
+  }
+  wave_sync();
+  return true;
+}
+
 // Huffman tree description (RFC 8878 4.2.1.1) of L.code into L.tree by the
 // whole wave; the same bytes as zcore::huf_write_tree. Weights and their
 // histogram come from ballots; the FSE table of the weights (table log 6) is
@@ -1215,10 +1338,7 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
         }
       }
     }
-  } else if (nlits >= 256 && used > 1) {
-    // used symbols in ascending count order: bitonic sort of (count, symbol)
-    huf_sort_wave(L, used);
-    huf_build_wave(L, used);
+  } else if (nlits >= 256 && used > 1 && huf_build_fast_wave(L)) {
     {
       const uint32_t ts = huf_write_tree_wave(L);
       if (lane == 0)
