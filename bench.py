@@ -22,11 +22,14 @@ reference CPU path is timed on the same input (cpu_baseline).
 Beside the headline (extra keys, never `value`): the same workload with the
 input already in HBM; the six-level 2x2x2 variant (two timepoints of 8^3
 chunks); 512 sensor frames of 4096x2304 (64x256x256 chunks, 1x8x9-chunk
-shards, zstd 1) from host memory; the 256^3 lz4 volume from host memory.
+shards, zstd 1) from host memory; the 256^3 lz4 volume from host memory; the
+headline workload written to a zarr filesystem store (page cache and
+O_DIRECT); the serving mix.
 
 Other workloads (--workload): `sensor`, `volume`, `multiscale6`, `lod`
 (2048^2 frames, 4-level mean LOD, no codec), `serving` (24 Poisson
-arrivals); `--device-input` times HBM-resident input instead.
+arrivals), `fs` (the filesystem-store leg alone); `--device-input` times
+HBM-resident input instead.
 """
 from __future__ import annotations
 
@@ -297,6 +300,83 @@ def leg(name, steps, warmup, host_input, frames=512, options=None):
     return out
 
 
+def mount_of(path):
+    """(mount point, filesystem type) holding `path`, from /proc/mounts."""
+    best = ("", "?")
+    try:
+        real = os.path.realpath(path)
+        for line in open("/proc/mounts"):
+            f = line.split()
+            if len(f) > 2 and (real == f[1] or real.startswith(f[1].rstrip("/") + "/")) and len(f[1]) >= len(best[0]):
+                best = (f[1], f[2])
+    except OSError:
+        pass
+    return best
+
+
+def fs_sink_leg(steps, warmup):
+    """The headline workload (host input, batched pipeline) written by the
+    zarr filesystem sink, each step into a fresh store as an acquisition
+    would be (the sink is created before the timer; the step is reset +
+    append + flush, and the flush waits for every shard write and the
+    metadata): through the page cache, and as the reference's benchmark
+    writes to disk (bench/bench_util.c:551-587: O_DIRECT shards, shard
+    alignment of a page, which keeps the stream on the plain pipeline). The
+    stores live in a temporary directory, removed afterwards."""
+    import shutil
+    import tempfile
+    import chucky_amd as ca
+    from chucky_amd.api import FsSink
+    _, shape, desc = workload("multiscale")
+    data, host = make_input("multiscale", shape, True)
+    out = {"workload": desc.replace("discard sink", "zarr filesystem store (OME-NGFF group)"),
+           "input_GB": round(data.nbytes / 1e9, 4), "input_from": "host memory",
+           "steps": steps, "warmup": warmup}
+    root = tempfile.mkdtemp(prefix="chucky_fs_")
+    mnt, fstype = mount_of(root)
+    out["store_filesystem"] = {"mount": mnt, "type": fstype}
+    page = os.sysconf("SC_PAGE_SIZE")
+    try:
+        for mode, unbuffered in (("page_cache", False), ("o_direct", True)):
+            c = workload("multiscale")[0]
+            if unbuffered:
+                c.shard_alignment = page
+            prev = FsSink(os.path.join(root, mode, "0"), c, name="vol", unbuffered=unbuffered)
+            s = ca.GpuStream(c, prev, pipeline=1, pass_timing=2)
+            ms, written, err = [], 0, None
+            for i in range(warmup + steps):
+                store = os.path.join(root, mode, str(i))
+                sink = prev if i == 0 else FsSink(store, c, name="vol", unbuffered=unbuffered)
+                ca.lib.chucky_device_synchronize()
+                t0 = time.perf_counter()
+                if s.reset(sink) or s.append(host) or s.flush():
+                    err = ca.last_error()
+                ca.lib.chucky_device_synchronize()
+                dt = (time.perf_counter() - t0) * 1e3
+                if prev is not sink:
+                    prev.close()  # the reset has let go of it
+                prev = sink
+                if err:
+                    break
+                if i >= warmup:
+                    ms.append(dt)
+                    written = sum(os.path.getsize(os.path.join(dp, f)) for dp, _, fs in os.walk(store) for f in fs)
+            s.close()
+            prev.close()
+            if err or not ms:
+                out[mode] = {"error": (err or "no steps")[:200]}
+                continue
+            mean = sorted(ms)[len(ms) // 2] if len(ms) > 4 else sum(ms) / len(ms)
+            out[mode] = {"GBps": round(data.nbytes / (mean * 1e-3) / 1e9, 3), "ms_per_step": round(mean, 3),
+                         "store_bytes": written, "shard_alignment": c.shard_alignment,
+                         "pipeline": "plain (page-aligned shards)" if c.shard_alignment else "batched",
+                         "step_ms": [round(v, 3) for v in ms]}
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+        data.free()
+    return out
+
+
 SERVING_DESC = ("serving: 24 Poisson arrivals (4/s) over u8 2048x512x512, u16 256x2048x2048, f32 256^3; "
                 "64^3 or 64x256x256 chunks; zstd/lz4; 1-5 LOD levels; 200 GB HBM budget; discard sinks")
 
@@ -432,7 +512,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=512, help="frames of the sensor / lod workloads")
-    ap.add_argument("--workload", choices=["multiscale", "multiscale6", "sensor", "volume", "lod", "serving"],
+    ap.add_argument("--workload", choices=["multiscale", "multiscale6", "sensor", "volume", "lod", "serving", "fs"],
                     default="multiscale")
     ap.add_argument("--pipeline", choices=["plain", "batched", "both"], default="both",
                     help="time one pipeline alone (profiling runs) or both")
@@ -462,6 +542,9 @@ def main():
         return
     if args.workload == "serving":
         print(json.dumps(serving_line(args)))
+        return
+    if args.workload == "fs":
+        print(json.dumps(fs_sink_leg(args.steps, args.warmup)))
         return
 
     import chucky_amd as ca
@@ -547,6 +630,11 @@ def main():
         extras["multiscale6_host_input"] = leg("multiscale6", max(3, args.steps // 4), 1, True)
         extras["volume_lz4_host_input"] = leg("volume", max(3, args.steps // 4), 1, True)
         extras["sensor_host_input"] = leg("sensor", 2, 1, True, frames=512)
+        try:
+            extras["fs_sink_host_input"] = fs_sink_leg(max(3, args.steps // 2), 2)
+            extras["fs_sink_host_input"]["discard_sink_GBps"] = line["value"]
+        except Exception as e:  # the headline stands on its own
+            extras["fs_sink_host_input"] = {"error": str(e)[:300]}
         try:
             extras["serving"] = serving(args)
         except Exception as e:  # the headline stands on its own

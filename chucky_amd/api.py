@@ -140,6 +140,67 @@ class MemSink:
         self.close()
 
 
+class _ZarrFsConfig(C.Structure):
+    _fields_ = [("store_path", C.c_char_p), ("array_name", C.c_char_p), ("data_type", C.c_int),
+                ("fill_value", C.c_double), ("rank", C.c_uint8),
+                ("dimensions", C.POINTER(Dimension)), ("unbuffered", C.c_int),
+                ("codec", CodecConfig)]
+
+
+class _ZarrFsMultiscaleConfig(C.Structure):
+    _fields_ = [("store_path", C.c_char_p), ("array_name", C.c_char_p), ("data_type", C.c_int),
+                ("fill_value", C.c_double), ("rank", C.c_uint8),
+                ("dimensions", C.POINTER(Dimension)), ("nlod", C.c_int), ("unbuffered", C.c_int),
+                ("codec", CodecConfig)]
+
+
+class FsSink:
+    """Zarr v3 filesystem store (zarr_fs_sink_* of zarr_fs_sink.h): one array
+    `name` under `store`, or an OME-NGFF multiscale group of arrays 0, 1, ...
+    when the stream downsamples (zarr_fs_multiscale_sink_*, levels from the
+    stream's configuration). `unbuffered` writes the shards with O_DIRECT
+    (the store must be on a filesystem that supports it)."""
+
+    def __init__(self, store: str, cfg: "Config", name: str = "arr", unbuffered: bool = False):
+        multiscale = any(d.downsample for d in cfg.dims)
+        self._pre = "zarr_fs_multiscale_sink" if multiscale else "zarr_fs_sink"
+        cfg_t = _ZarrFsMultiscaleConfig if multiscale else _ZarrFsConfig
+        fn = getattr(lib, self._pre + "_create")
+        fn.restype = C.c_void_p
+        fn.argtypes = [C.POINTER(cfg_t)]
+        getattr(lib, self._pre + "_as_shard_sink").restype = C.c_void_p
+        getattr(lib, self._pre + "_as_shard_sink").argtypes = [C.c_void_p]
+        getattr(lib, self._pre + "_destroy").argtypes = [C.c_void_p]
+        c = cfg.to_c()
+        z = cfg_t()
+        z.store_path = store.encode()
+        z.array_name = name.encode()
+        z.data_type = c.dtype
+        z.fill_value = 0.0
+        z.rank = c.rank
+        z.dimensions = c.dimensions
+        z.unbuffered = int(unbuffered)
+        z.codec = c.codec
+        if multiscale:
+            z.nlod = 0
+        self._keep = (c, z, cfg)
+        self.h = fn(C.byref(z))
+        if not self.h:
+            raise RuntimeError("fs sink: " + last_error())
+
+    @property
+    def shard_sink(self):
+        return getattr(lib, self._pre + "_as_shard_sink")(self.h)
+
+    def close(self):
+        if self.h:
+            getattr(lib, self._pre + "_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class DiscardSink:
     def __init__(self):
         self.h = lib.chucky_discard_sink_create()

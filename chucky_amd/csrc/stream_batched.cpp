@@ -890,8 +890,19 @@ int batched_drain(tile_stream_gpu* s)
 
 void batched_reset(tile_stream_gpu* s)
 {
-  if (s->batched)
-    s->batched->next_epoch = 0;
+  BatchedState* b = s->batched;
+  if (!b)
+    return;
+  b->next_epoch = 0;
+  // The host slots' fences belong to the sink of the previous acquisition:
+  // its writes out of the slots end before the slots are reused, and a reset
+  // may hand the stream another sink, whose fences count from zero.
+  if (s->sink && s->sink->wait_fence)
+    for (auto& H : b->host)
+      for (int lv = 0; lv < s->plan.nlod; ++lv)
+        s->sink->wait_fence(s->sink, (uint8_t)lv, H.fence[lv]);
+  for (auto& H : b->host)
+    std::fill(H.fence.begin(), H.fence.end(), io_event{ 0 });
 }
 
 } // namespace chucky

@@ -1180,6 +1180,13 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
 {
   if (hipSetDevice(s->device) != hipSuccess || hipStreamSynchronize(s->st_comp) != hipSuccess)
     return 1;
+  // writes of the previous acquisition out of the stream's pinned buffers
+  // end before those buffers are reused or the sink is replaced (its fences
+  // count in its own sequence)
+  if (s->sink && s->sink->wait_fence)
+    for (int lv = 0; lv < s->plan.nlod; ++lv)
+      s->sink->wait_fence(s->sink, (uint8_t)lv, s->io_done[lv]);
+  batched_reset(s);
   if (sink)
     s->sink = sink;
   s->cursor = 0;
@@ -1198,7 +1205,6 @@ int tile_stream_gpu_reset(struct tile_stream_gpu* s, struct shard_sink* sink)
   std::fill(s->io_done.begin(), s->io_done.end(), io_event{ 0 });
   s->latencies.clear();
   s->epochs_done = s->epochs_written = 0;
-  batched_reset(s);
   return 0;
 }
 

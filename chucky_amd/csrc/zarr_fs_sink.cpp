@@ -550,17 +550,32 @@ int fs_write_direct(shard_writer* self, uint64_t offset, const void* beg, const 
 {
   FsWriter* w = reinterpret_cast<FsWriter*>(self);
   zarr_fs_sink* zs = w->owner;
-  // O_DIRECT has the kernel DMA from the caller's pages. The stream hands
-  // over pinned buffers from the HIP runtime, which are not ordinary user
-  // pages: O_DIRECT writes from them came out as zeros or failed, so
-  // unbuffered stores take the bounce copy into an aligned buffer.
-  if (zs->unbuffered)
-    return fs_write(self, offset, beg, end);
   const size_t n = (size_t)((const char*)end - (const char*)beg);
   if (n == 0)
     return 0;
   const int fd = w->fd;
   zs->queued += n;
+  // O_DIRECT has the kernel DMA from the caller's pages. The stream hands
+  // over pinned buffers from the HIP runtime, which are not ordinary user
+  // pages: O_DIRECT writes from them came out as zeros or failed, so
+  // unbuffered stores write from an aligned bounce copy -- made by the I/O
+  // worker, since the caller's buffer stays valid until the next fence.
+  if (zs->unbuffered) {
+    const size_t page = zs->page;
+    zs->queue->post([zs, fd, beg, n, offset, page] {
+      void* copy = nullptr;
+      if (posix_memalign(&copy, page, n)) {
+        io_failed(zs, ENOMEM, beg, n, offset);
+      } else {
+        std::memcpy(copy, beg, n);
+        if (const int e = pwrite_all(fd, copy, n, offset))
+          io_failed(zs, e, copy, n, offset);
+        std::free(copy);
+      }
+      zs->retired += n;
+    });
+    return io_status(zs);
+  }
   zs->queue->post([zs, fd, beg, n, offset] {
     if (const int e = pwrite_all(fd, beg, n, offset))
       io_failed(zs, e, beg, n, offset);

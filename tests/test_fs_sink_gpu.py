@@ -117,3 +117,31 @@ def test_gpu_stream_into_fs_sink(tmp_path, name, pipeline):
     if unbuffered:
         import shutil
         shutil.rmtree(base, ignore_errors=True)
+
+
+@pytest.mark.parametrize("pipeline", [ca.PIPELINE_PLAIN, ca.PIPELINE_BATCHED])
+def test_reset_into_another_fs_sink(tmp_path, pipeline):
+    # one stream, two acquisitions: the second reset hands it a new store.
+    # The new sink's write fences count from zero, so none of the first
+    # sink's fences may be waited for on it; the first sink is destroyed only
+    # after the reset has let go of it. Both stores come out identical.
+    multiscale, dims, dtype, codec, _ = CASES["multiscale_zstd"]
+    cfg = ca.Config(dims, dtype=dtype, codec=codec, buffer_capacity_bytes=1 << 16)
+    data = _data(dims, dtype)
+    print(f"[fs sink reset] pipeline={pipeline}", flush=True)
+    stores = [str(tmp_path / "a"), str(tmp_path / "b")]
+    sinks = [_Sink(ca.lib, p, cfg, multiscale) for p in stores]
+    s = ca.GpuStream(cfg, sinks[0], pipeline=pipeline)
+    for k, sink in enumerate(sinks):
+        if k:
+            assert s.reset(sink) == 0, ca.last_error()
+            sinks[k - 1].close()
+        for part in np.array_split(data, 3):
+            assert s.append(part) == 0, ca.last_error()
+        assert s.flush() == 0, ca.last_error()
+    s.close()
+    sinks[-1].close()
+    a, b = _tree(stores[0]), _tree(stores[1])
+    assert sorted(a) == sorted(b) and a
+    for path in a:
+        assert a[path] == b[path], path
