@@ -104,7 +104,10 @@ def run(cfg, desc, data, host, steps, warmup, pipelines, host_input, options=Non
     import chucky_amd as ca
     nbytes = data.nbytes
     streams = {}
-    for p in pipelines:
+    # the batched pipeline's streams are created first: HIP hands its hardware
+    # queues to streams in creation order, and the batched pipeline's
+    # kernel-launching streams are laid out to take distinct ones
+    for p in sorted(pipelines, key=lambda q: q != "batched"):
         sink = ca.DiscardSink()
         streams[p] = (ca.GpuStream(cfg, sink, pipeline=1 if p == "batched" else 0, **(options or {})), sink)
 
@@ -553,10 +556,12 @@ def main():
                "zstd_chunk_tables": args.zstd_chunk_tables, "pass_timing": args.pass_timing}
     cfg, shape, desc = workload(args.workload, args.frames)
     data, host = make_input(args.workload, shape, True)
-    h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)
-    d2h_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 0)
     pipes = ["plain", "batched"] if args.pipeline == "both" else [args.pipeline]
     runs = run(cfg, desc, data, host, args.steps, args.warmup, pipes, host_input, options)
+    # the host link's peak, measured after the timed steps: seconds of
+    # back-to-back 256 MiB copies leave the link slower for a while
+    h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)
+    d2h_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 0)
     lead = "batched" if "batched" in runs else "plain"
     value, head = summarize(runs[lead], args.steps)
     roofline = compress_roofline(runs[lead], args.workload)
