@@ -558,8 +558,8 @@ def main():
     data, host = make_input(args.workload, shape, True)
     pipes = ["plain", "batched"] if args.pipeline == "both" else [args.pipeline]
     runs = run(cfg, desc, data, host, args.steps, args.warmup, pipes, host_input, options)
-    # the host link's peak, measured after the timed steps: seconds of
-    # back-to-back 256 MiB copies leave the link slower for a while
+    # the host link's peak, measured after the timed steps, so that no other
+    # pinned allocation precedes the streams' own
     h2d_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 1)
     d2h_peak = ca.lib.chucky_measure_link_gbps(256 << 20, 8, 0)
     lead = "batched" if "batched" in runs else "plain"
