@@ -184,31 +184,54 @@ __device__ __forceinline__ void count_bytes_block(const uint8_t* src, uint64_t n
     atomicAdd(&hist[src[i]], 1u);
 }
 
-// The up-to-8 symbols lane `lane` encodes in the round whose highest symbol
-// is `top`: symbols top-8*lane-7 .. top-8*lane, as bytes of a 64-bit word
-// (byte j = symbol lo+j; bytes below `a` are zero and skipped by the caller).
-__device__ __forceinline__ uint64_t round_syms(const uint8_t* lits, long a, long top, int lane,
-                                               int* skip)
+// The raw dwords holding the up-to-8 symbols lane `lane` encodes in the round
+// whose highest symbol is `top` (symbols top-8*lane-7 .. top-8*lane; those
+// below `a` are skipped). The load is unconditional (a lane with nothing to
+// encode reads at `a`) and its bytes are only aligned when the round uses
+// them, so it stays in flight over the previous round.
+struct RoundLoad
+{
+  uint32_t w0, w1, w2, sh;
+  int skip;
+};
+
+// The literals' memory, named for the compiler: global (address space 1) or
+// LDS (3), so that the load is a global or an LDS load and not a flat one (a
+// flat load counts against the LDS counter too, so waiting for the round's
+// table lookups would also wait for it).
+template<int AS> using LitPtr = const __attribute__((address_space(AS))) uint8_t*;
+template<int AS> using LitWord = const __attribute__((address_space(AS))) uint32_t*;
+
+template<int AS>
+__device__ __forceinline__ RoundLoad round_load(LitPtr<AS> lits, long a, long top, int lane)
 {
   const long lo = top - (long)lane * kSymPerLane - (kSymPerLane - 1);
-  if (lo + kSymPerLane - 1 < a) {
-    *skip = kSymPerLane;
-    return 0;
-  }
-  const int sk = lo >= a ? 0 : (int)(a - lo);
-  *skip = sk;
-  return ld64u(lits + lo + sk) << (8 * sk);
+  const int sk = lo + kSymPerLane - 1 < a ? kSymPerLane : (lo >= a ? 0 : (int)(a - lo));
+  const long at = sk >= kSymPerLane ? a : lo + sk;
+  const uint32_t mis = (uint32_t)(uintptr_t)(lits + at) & 3u;
+  const LitWord<AS> w = (LitWord<AS>)(lits + (at - (long)mis));
+  return RoundLoad{ w[0], w[1], w[2], mis * 8, sk };
+}
+
+// byte j of the result = symbol lo + j; skipped symbols are zero
+__device__ __forceinline__ uint64_t round_syms(const RoundLoad& r)
+{
+  const uint32_t lo = __builtin_amdgcn_alignbit(r.w1, r.w0, r.sh);
+  const uint32_t hi = __builtin_amdgcn_alignbit(r.w2, r.w1, r.sh);
+  const uint64_t v = ((uint64_t)hi << 32) | lo;
+  return r.skip >= kSymPerLane ? 0ull : v << (8 * r.skip);
 }
 
 // Encodes lits[a, b) as one backward Huffman stream at out; returns its size.
 // Each round the wave takes 512 symbols, highest first: every lane packs its
 // 8 codes (at most 88 bits) into two 64-bit accumulators, a wave prefix sum
 // of the bit counts places the lanes' words, funnel shifts align them and
-// LDS atomic-ORs assemble them in `stage`; complete bytes are copied out
-// once the stage could not take another round. The next round's symbols are
-// loaded before this round's codes are looked up, so the load latency
-// overlaps the LDS work.
-__device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t a, uint32_t b,
+// LDS atomic-ORs assemble them in `stage` (a lane ORs all four words, zero
+// ones included, so no lane branches); complete bytes are copied out once
+// the stage could not take another round. The next round's symbols are
+// loaded while this round's codes are looked up and placed.
+template<int AS>
+__device__ uint32_t huf_stream_wave(uint8_t* out, LitPtr<AS> lits, uint32_t a, uint32_t b,
                                     const uint32_t* cw, uint32_t* stage)
 {
   static_assert(5 * zcore::kHufMaxBits <= 64, "five codes fit one accumulator");
@@ -219,26 +242,33 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
   uint32_t flushed = 0; // bytes already copied out
   uint32_t rel = 0;     // bits held in stage (stage bit 0 = stream bit 8*flushed)
   long top = (long)b - 1;
-  int skip = 0;
-  uint64_t q = round_syms(lits, a, top, lane, &skip);
+  RoundLoad cur = round_load<AS>(lits, a, top, lane);
   for (;;) {
     const long next = top - (long)kWave * kSymPerLane;
-    int nskip = kSymPerLane;
-    const uint64_t nq = next >= (long)a ? round_syms(lits, a, next, lane, &nskip) : 0;
+    const RoundLoad nxt = round_load<AS>(lits, a, next >= (long)a ? next : (long)a - 1, lane);
+    const uint64_t q = round_syms(cur);
+    const int skip = cur.skip;
+    // every lookup is made (a skipped symbol reads the entry of 0) and its
+    // code dropped afterwards: no exec-mask changes around the LDS reads
+    uint32_t c[kSymPerLane];
+#pragma unroll
+    for (int j = 0; j < kSymPerLane; ++j)
+      c[j] = cw[(uint32_t)(q >> (8 * j)) & 0xFF];
+#pragma unroll
+    for (int j = 0; j < kSymPerLane; ++j)
+      c[j] = j >= skip ? c[j] : 0u;
     // symbols 7..3 into A, 2..0 into B (a skipped symbol adds no bits)
     uint64_t A = 0, B = 0;
     uint32_t na = 0, nbb = 0;
 #pragma unroll
     for (int j = kSymPerLane - 1; j >= 3; --j) {
-      const uint32_t c = j >= skip ? cw[(uint32_t)(q >> (8 * j)) & 0xFF] : 0u;
-      A |= (uint64_t)(c & 0xFFFF) << na;
-      na += c >> 16;
+      A |= (uint64_t)(c[j] & 0xFFFF) << na;
+      na += c[j] >> 16;
     }
 #pragma unroll
     for (int j = 2; j >= 0; --j) {
-      const uint32_t c = j >= skip ? cw[(uint32_t)(q >> (8 * j)) & 0xFF] : 0u;
-      B |= (uint64_t)(c & 0xFFFF) << nbb;
-      nbb += c >> 16;
+      B |= (uint64_t)(c[j] & 0xFFFF) << nbb;
+      nbb += c[j] >> 16;
     }
     const uint32_t nb = na + nbb;
     // the lane's bits: lo | hi << 64, hi below 2^24
@@ -246,7 +276,9 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
     const uint32_t hi = na ? (uint32_t)(B >> (64 - na)) : 0u;
     const uint32_t incl = wave_incl_scan(nb);
     const uint32_t total = wave_bcast(incl, kWave - 1);
-    if (nb) {
+    {
+      // rel + total stays below the flush bound plus one round: the four
+      // words lie inside the stage for every lane, nb = 0 included
       const uint32_t p = rel + incl - nb;
       const uint32_t sh = p & 31;
       const uint32_t v0 = (uint32_t)lo, v1 = (uint32_t)(lo >> 32);
@@ -255,14 +287,10 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
       const uint32_t w2 = sh ? __builtin_amdgcn_alignbit(hi, v1, 32 - sh) : hi;
       const uint32_t w3 = sh ? hi >> (32 - sh) : 0u;
       uint32_t* w = stage + (p >> 5);
-      if (w0)
-        atomicOr(&w[0], w0);
-      if (w1)
-        atomicOr(&w[1], w1);
-      if (w2)
-        atomicOr(&w[2], w2);
-      if (w3)
-        atomicOr(&w[3], w3);
+      atomicOr(&w[0], w0);
+      atomicOr(&w[1], w1);
+      atomicOr(&w[2], w2);
+      atomicOr(&w[3], w3);
     }
     rel += total;
     if (next < (long)a) {
@@ -289,8 +317,7 @@ __device__ uint32_t huf_stream_wave(uint8_t* out, const uint8_t* lits, uint32_t 
       rel &= 7;
     }
     top = next;
-    q = nq;
-    skip = nskip;
+    cur = nxt;
   }
 }
 
@@ -1146,9 +1173,11 @@ __device__ uint32_t write_sequences_wave(Lds& L, uint8_t* out, const Seq* seqs, 
 }
 
 // Compresses one block; returns bytes written at out (header included).
+// in_lds: `in` is the LDS-staged block (else global memory); the literal
+// scratch `lits` is global memory.
 __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, uint32_t len, uint8_t* out, bool last,
                                    uint8_t* lits, Seq* seqs, uint8_t* seqout, uint32_t* aux,
-                                   const zcore::Predefined& pre, const ChunkTable* tab)
+                                   const zcore::Predefined& pre, const ChunkTable* tab, bool in_lds)
 {
   const int lane = lane_id();
 
@@ -1291,6 +1320,8 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
       return 4;
     }
   }
+  // the literals are read from the block itself when it has no matches
+  const bool lits_lds = nseq == 0 && in_lds;
   if (nseq == 0) {
     count_bytes(in, len, L.hist);
     lits = const_cast<uint8_t*>(in);
@@ -1402,7 +1433,8 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
     for (int k = 0; k < 4; ++k) {
       const uint32_t a = (uint32_t)k * seg;
       const uint32_t b = k == 3 ? nlits : a + seg;
-      sizes[k] = huf_stream_wave(o + sp, lits, a, b, L.u.enc.cw, L.u.enc.stage);
+      sizes[k] = lits_lds ? huf_stream_wave<3>(o + sp, (LitPtr<3>)lits, a, b, L.u.enc.cw, L.u.enc.stage)
+                          : huf_stream_wave<1>(o + sp, (LitPtr<1>)lits, a, b, L.u.enc.cw, L.u.enc.stage);
       sp += sizes[k];
     }
     const uint32_t comp = sp - hs;
@@ -1581,7 +1613,7 @@ __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src,
       blk = W.in;
     }
     const uint32_t sz = compress_block(L, blk + skip, len - skip, out + hdr, b + 1 == per_chunk, lits, seqs,
-                                       seqout, aux, *pre, tab);
+                                       seqout, aux, *pre, tab, Staged);
     if (lane == 0)
       seg_sizes[chunk * per_chunk + b] = hdr + sz;
     __syncthreads();
