@@ -118,7 +118,7 @@ __device__ __forceinline__ uint32_t key_hash(uint64_t key)
 // (symbol, run) pair per byte position within a dword and only issues an LDS
 // atomic when the symbol changes: the high bytes of small 16/32-bit integers
 // (most often 0) would otherwise make every atomic of the wave hit one bin.
-__device__ __forceinline__ void count_bytes(const uint8_t* src, uint32_t n, uint32_t* hist)
+__device__ __forceinline__ void count_bytes_any(const uint8_t* src, uint32_t n, uint32_t* hist)
 {
   const int lane = lane_id();
   uint32_t sym[4] = { 0, 0, 0, 0 }, run[4] = { 0, 0, 0, 0 };
@@ -149,6 +149,48 @@ __device__ __forceinline__ void count_bytes(const uint8_t* src, uint32_t n, uint
       atomicAdd(&hist[sym[k]], run[k]);
   for (uint32_t i = body + lane; i < n; i += kWave)
     atomicAdd(&hist[src[i]], 1u);
+}
+
+template<int AS> using BytePtr = const __attribute__((address_space(AS))) uint8_t*;
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+template<int AS> using Vec4Ptr = const __attribute__((address_space(AS))) v4u32*;
+
+// The same for a 16-byte aligned src in global memory (AS 1) or LDS (3): one
+// 16-byte load per lane and round, the next round's in flight while this
+// one's bytes are counted; the part below a whole round by count_bytes_any.
+template<int AS>
+__device__ __forceinline__ void count_bytes(BytePtr<AS> src, uint32_t n, uint32_t* hist)
+{
+  const int lane = lane_id();
+  uint32_t sym[4] = { 0, 0, 0, 0 }, run[4] = { 0, 0, 0, 0 };
+  const uint32_t body = n & ~1023u; // whole rounds of 16 bytes per lane
+  v4u32 cur = body ? ((Vec4Ptr<AS>)src)[lane] : v4u32{ 0, 0, 0, 0 };
+  for (uint32_t i = 16u * lane; i < body; i += 16u * kWave) {
+    const uint32_t nx = i + 16u * kWave;
+    const v4u32 nxt = ((Vec4Ptr<AS>)(src + (nx < body ? nx : i)))[0];
+    const uint32_t w[4] = { cur.x, cur.y, cur.z, cur.w };
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t b = (w[j] >> (8 * k)) & 0xFF;
+        if (b == sym[k]) {
+          ++run[k];
+        } else {
+          if (run[k])
+            atomicAdd(&hist[sym[k]], run[k]);
+          sym[k] = b;
+          run[k] = 1;
+        }
+      }
+    cur = nxt;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (run[k])
+      atomicAdd(&hist[sym[k]], run[k]);
+  if (body < n)
+    count_bytes_any((const uint8_t*)(src + body), n - body, hist);
 }
 
 // count_bytes for a whole workgroup (blockDim.x threads).
@@ -1323,7 +1365,12 @@ __device__ __forceinline__ uint32_t compress_block(Lds& L, const uint8_t* in, ui
   // the literals are read from the block itself when it has no matches
   const bool lits_lds = nseq == 0 && in_lds;
   if (nseq == 0) {
-    count_bytes(in, len, L.hist);
+    if (((uintptr_t)in & 15) != 0)
+      count_bytes_any(in, len, L.hist);
+    else if (in_lds)
+      count_bytes<3>((BytePtr<3>)in, len, L.hist);
+    else
+      count_bytes<1>((BytePtr<1>)in, len, L.hist);
     lits = const_cast<uint8_t*>(in);
   } else {
     gather_literals(in, len, seqs, nseq, (uint32_t*)L.u.table, lits, L.hist);
