@@ -40,16 +40,35 @@ __device__ __forceinline__ uint64_t item_bytes(const SegmentSource& s, uint32_t 
 
 __device__ __forceinline__ void plan_body(
   const SegmentSource& src, const uint32_t* __restrict__ gather, const uint32_t* __restrict__ perm,
-  uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* chunk_sizes,
-  uint64_t* offsets, uint64_t* host_sizes, uint64_t* host_offsets)
+  uint64_t n_items, uint64_t n_slots, uint64_t group, uint64_t page, uint64_t* __restrict__ chunk_sizes,
+  uint64_t* __restrict__ offsets, uint64_t* __restrict__ host_sizes, uint64_t* __restrict__ host_offsets)
 {
+  // Every pass issues the loads of kBatch slots before using them: a pass
+  // that loaded, used and stored one slot at a time would wait a global
+  // memory latency per slot (tens of slots per thread for small chunks).
+  constexpr int kBatch = 8;
   __shared__ uint64_t part[kPlanThreads];
   const int t = threadIdx.x;
   for (uint64_t i = t; i < n_slots; i += kPlanThreads)
     chunk_sizes[i] = 0;
   __syncthreads();
-  for (uint64_t i = t; i < n_items; i += kPlanThreads)
-    chunk_sizes[perm[i]] = item_bytes(src, gather[i]);
+  for (uint64_t i0 = t; i0 < n_items; i0 += (uint64_t)kPlanThreads * kBatch) {
+    uint32_t g[kBatch], pm[kBatch];
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) {
+      const uint64_t i = i0 + (uint64_t)q * kPlanThreads;
+      g[q] = i < n_items ? gather[i] : 0u;
+      pm[q] = i < n_items ? perm[i] : 0u;
+    }
+    uint64_t v[kBatch];
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      v[q] = item_bytes(src, g[q]);
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      if (i0 + (uint64_t)q * kPlanThreads < n_items)
+        chunk_sizes[pm[q]] = v[q];
+  }
   __syncthreads();
 
   // Contiguous range per thread; padding is added on the fly to the value of
@@ -57,8 +76,7 @@ __device__ __forceinline__ void plan_body(
   const uint64_t per = (n_slots + kPlanThreads - 1) / kPlanThreads;
   const uint64_t b = (uint64_t)t * per;
   const uint64_t e = b + per < n_slots ? b + per : n_slots;
-  auto value = [&](uint64_t i) -> uint64_t {
-    uint64_t v = chunk_sizes[i];
+  auto pad = [&](uint64_t i, uint64_t v) -> uint64_t {
     if (page && group && (i % group) == group - 1) {
       uint64_t tot = 0;
       for (uint64_t j = i + 1 - group; j <= i; ++j)
@@ -68,8 +86,15 @@ __device__ __forceinline__ void plan_body(
     return v;
   };
   uint64_t local = 0;
-  for (uint64_t i = b; i < e; ++i)
-    local += value(i);
+  for (uint64_t i0 = b; i0 < e; i0 += kBatch) {
+    uint64_t v[kBatch];
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      v[q] = i0 + q < e ? chunk_sizes[i0 + q] : 0ull;
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      local += i0 + q < e ? pad(i0 + q, v[q]) : 0ull;
+  }
   part[t] = local;
   __syncthreads();
   // Hillis-Steele inclusive scan over the 1024 partials.
@@ -80,18 +105,32 @@ __device__ __forceinline__ void plan_body(
     __syncthreads();
   }
   uint64_t run = part[t] - local;
-  for (uint64_t i = b; i < e; ++i) {
-    offsets[i] = run;
-    if (host_offsets)
-      host_offsets[i] = run;
-    if (host_sizes)
-      host_sizes[i] = chunk_sizes[i];
-    run += value(i);
+  for (uint64_t i0 = b; i0 < e; i0 += kBatch) {
+    uint64_t v[kBatch];
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      v[q] = i0 + q < e ? chunk_sizes[i0 + q] : 0ull;
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q)
+      if (i0 + q < e) {
+        offsets[i0 + q] = run;
+        run += pad(i0 + q, v[q]);
+      }
   }
-  if (t == kPlanThreads - 1) {
+  if (t == kPlanThreads - 1)
     offsets[n_slots] = part[t];
-    if (host_offsets)
-      host_offsets[n_slots] = part[t];
+  if (host_offsets || host_sizes) {
+    // the copies for the host go out with consecutive threads on consecutive
+    // slots: a thread's own contiguous range would write pinned memory at a
+    // 8 * per byte stride across the wavefront, one PCIe transaction per
+    // 8-byte value
+    __syncthreads();
+    for (uint64_t i = t; i <= n_slots; i += kPlanThreads) {
+      if (host_offsets)
+        host_offsets[i] = offsets[i];
+      if (host_sizes && i < n_slots)
+        host_sizes[i] = chunk_sizes[i];
+    }
   }
 }
 
