@@ -802,50 +802,50 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
     hsz = wave_bcast(hsz, 0);
     // the weights' two-state stream, backward, even positions on state `se`.
     // Every weight's symbol transform is gathered first (one lane-indexed
-    // shuffle per register), so a step of the state chains -- which run in
-    // scalar registers -- reads one lane for its transform and one for its
-    // next state; the step's (bits, count) record goes into the lane of the
-    // weight's own index with a lane write (no LDS store, no exec mask).
+    // shuffle per register) and split into its two fields, so a step of the
+    // state chains -- which run in scalar registers -- reads two lanes for
+    // the transform and one for its next state, and records only the state it
+    // leaves (a lane write into the lane of the weight's own index, no LDS
+    // store, no exec mask); the (bits, count) records follow from the states
+    // afterwards, one lane per weight.
     auto rl = [](uint32_t v, uint32_t i) -> uint32_t {
       return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
     };
-    uint32_t ew[4], rec[4] = { 0u, 0u, 0u, 0u };
+    uint32_t ed[4], ef[4], rec[4] = { 0u, 0u, 0u, 0u };
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      ew[k] = (uint32_t)__shfl((int)symv, (int)(wt[k] & 63u), kWave);
-    auto tr = [&](int i) -> uint32_t {
-      const uint32_t v = (i >> 6) == 0 ? ew[0] : (i >> 6) == 1 ? ew[1] : (i >> 6) == 2 ? ew[2] : ew[3];
-      return rl(v, (uint32_t)(i & 63));
-    };
-    auto init = [&](uint32_t e) -> uint32_t {
-      const uint32_t d = e & 0xFFFFFu;
-      const int f = (int)(e >> 20) - 128;
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t e = (uint32_t)__shfl((int)symv, (int)(wt[k] & 63u), kWave);
+      ed[k] = e & 0xFFFFFu;
+      ef[k] = (e >> 20) - 128u; // signed, modulo 2^32
+    }
+    auto init = [&](int i) -> uint32_t {
+      const int k = i >> 6;
+      const uint32_t d = rl(k == 0 ? ed[0] : k == 1 ? ed[1] : k == 2 ? ed[2] : ed[3], (uint32_t)(i & 63));
+      const uint32_t f = rl(k == 0 ? ef[0] : k == 1 ? ef[1] : k == 2 ? ef[2] : ef[3], (uint32_t)(i & 63));
       const uint32_t nb = (d + (1u << 15)) >> 16;
       const uint32_t v = (nb << 16) - d;
-      return rl(stv, (uint32_t)((int)(v >> nb) + f));
+      return rl(stv, (v >> nb) + f);
     };
     int i = nw - 1;
     uint32_t se, so;
     if (i & 1) {
-      so = init(tr(i));
-      se = init(tr(i - 1));
+      so = init(i);
+      se = init(i - 1);
     } else {
-      se = init(tr(i));
-      so = init(tr(i - 1));
+      se = init(i);
+      so = init(i - 1);
     }
     i -= 2;
     const int top = i; // records at weight indices top .. 0, in stream order
-    // one step of a state chain over weight index `wi`: its (bits, count)
-    // record into lane wi % 64 of rk, the next state returned
-    auto step = [&](uint32_t ek, uint32_t& rk, int wi, uint32_t st) -> uint32_t {
-      const uint32_t e = rl(ek, (uint32_t)(wi & 63));
-      const uint32_t d = e & 0xFFFFFu;
-      const int f = (int)(e >> 20) - 128;
+    // one step of a state chain over weight index `wi`: the state it leaves
+    // into lane wi % 64 of rk, the next state returned
+    auto step = [&](uint32_t dk, uint32_t fk, uint32_t& rk, int wi, uint32_t st) -> uint32_t {
+      const uint32_t li = (uint32_t)(wi & 63);
+      const uint32_t d = rl(dk, li), f = rl(fk, li);
       const uint32_t nb = (st + d) >> 16;
-      const uint32_t rv = ((st & ((1u << nb) - 1u)) << 3) | nb, rl_i = (uint32_t)(wi & 63);
       // (one scalar source per instruction on this target: the lane goes in m0)
-      asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(rk) : "s"(rv), "{m0}"(rl_i));
-      return rl(stv, (uint32_t)((int)(st >> nb) + f));
+      asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(rk) : "s"(st), "{m0}"(li));
+      return rl(stv, (st >> nb) + f);
     };
     // Odd indices run on `so`, even ones on `se`. After a first even step
     // (when top is even) the steps go in (odd, even) pairs, whose two chains
@@ -853,18 +853,20 @@ __device__ uint32_t huf_write_tree_wave(Lds& L)
     // 64-index boundaries are even).
 #pragma unroll
     for (int k = 3; k >= 0; --k) {
-      const uint32_t ek = ew[k];
+      const uint32_t dk = ed[k], fk = ef[k];
       uint32_t rk = rec[k];
       if (i >= 64 * k && !(i & 1)) {
-        se = step(ek, rk, i, se);
+        se = step(dk, fk, rk, i, se);
         --i;
       }
       for (; i >= 64 * k + 1; i -= 2) {
-        const uint32_t no = step(ek, rk, i, so);
-        se = step(ek, rk, i - 1, se);
+        const uint32_t no = step(dk, fk, rk, i, so);
+        se = step(dk, fk, rk, i - 1, se);
         so = no;
       }
-      rec[k] = rk;
+      // the record of each weight: (bits << 3) | count from the state it left
+      const uint32_t nb = (rk + ed[k]) >> 16;
+      rec[k] = ((rk & ((1u << nb) - 1u)) << 3) | nb;
     }
     const uint32_t nsteps = (uint32_t)(top + 1);
     // pack: lane l takes steps 4l .. 4l+3 (nsteps <= 253), step j being the
@@ -1098,15 +1100,16 @@ __device__ uint32_t write_sequences_wave(Lds& L, uint8_t* out, const Seq* seqs, 
     const uint32_t v = (nb << 16) - dnb;
     return rl(stv, (uint32_t)((int)(v >> nb) + dfs));
   };
-  // one encode step: returns (bits << 3) | count, advances the state
-  auto step = [&](uint32_t symv, uint32_t stv, uint32_t code, uint32_t& st) -> uint32_t {
-    const uint32_t e = rl(symv, code);
-    const uint32_t dnb = e & 0xFFFFFu;
-    const int dfs = (int)(e >> 20) - 128;
-    const uint32_t nb = (st + dnb) >> 16;
-    const uint32_t bits = st & ((1u << nb) - 1u);
-    st = rl(stv, (uint32_t)((int)(st >> nb) + dfs));
-    return (bits << 3) | nb;
+  // one encode step with the symbol's transform fields (d, f): advances the
+  // state; the state it leaves is recorded by the caller
+  auto adv = [&](uint32_t stv, uint32_t d, uint32_t f, uint32_t st) -> uint32_t {
+    const uint32_t nb = (st + d) >> 16;
+    return rl(stv, (st >> nb) + f);
+  };
+  // (bits << 3) | count that state st emits for a symbol with field d
+  auto emitted = [](uint32_t st, uint32_t d) -> uint32_t {
+    const uint32_t nb = (st + d) >> 16;
+    return ((st & ((1u << nb) - 1u)) << 3) | nb;
   };
   uint32_t sll, sml, sof;
   {
@@ -1115,21 +1118,32 @@ __device__ uint32_t write_sequences_wave(Lds& L, uint8_t* out, const Seq* seqs, 
     sof = init(of_sym, of_st, c >> 16);
     sll = init(ll_sym, ll_st, c & 0xFF);
   }
+  // Per round of 64 sequences every lane first looks up its own sequence's
+  // three symbol transforms and splits them into fields (dnb and dfs), so a
+  // step reads two lanes and the state table; the three states a sequence
+  // leaves go into its lane by lane writes, and the (bits, count) fields
+  // follow from them afterwards, one lane per sequence.
   for (long base = (long)nseq - 2; base >= 0; base -= kWave) {
     const long my = base - lane;
     const uint32_t v = my >= 0 ? aux[my] : 0u;
-    uint32_t outv = 0;
+    const uint32_t eo = (uint32_t)__shfl((int)of_sym, (int)(v >> 16), kWave);
+    const uint32_t em = (uint32_t)__shfl((int)ml_sym, (int)((v >> 8) & 0xFF), kWave);
+    const uint32_t el = (uint32_t)__shfl((int)ll_sym, (int)(v & 0xFF), kWave);
+    const uint32_t d_o = eo & 0xFFFFFu, d_m = em & 0xFFFFFu, d_l = el & 0xFFFFFu;
+    const uint32_t f_o = (eo >> 20) - 128u, f_m = (em >> 20) - 128u, f_l = (el >> 20) - 128u;
+    uint32_t r_o = 0, r_m = 0, r_l = 0;
     const int cnt = base + 1 < kWave ? (int)base + 1 : kWave;
     for (int j = 0; j < cnt; ++j) {
-      const uint32_t c = rl(v, (uint32_t)j);
-      const uint32_t fo = step(of_sym, of_st, c >> 16, sof);
-      const uint32_t fm = step(ml_sym, ml_st, (c >> 8) & 0xFF, sml);
-      const uint32_t fl = step(ll_sym, ll_st, c & 0xFF, sll);
-      if (lane == j)
-        outv = fo | (fm << 9) | (fl << 18);
+      // (one scalar source per instruction on this target: the lane goes in m0)
+      asm volatile("v_writelane_b32 %0, %3, m0\n\tv_writelane_b32 %1, %4, m0\n\tv_writelane_b32 %2, %5, m0"
+                   : "+v"(r_o), "+v"(r_m), "+v"(r_l)
+                   : "s"(sof), "s"(sml), "s"(sll), "{m0}"(j));
+      sof = adv(of_st, rl(d_o, (uint32_t)j), rl(f_o, (uint32_t)j), sof);
+      sml = adv(ml_st, rl(d_m, (uint32_t)j), rl(f_m, (uint32_t)j), sml);
+      sll = adv(ll_st, rl(d_l, (uint32_t)j), rl(f_l, (uint32_t)j), sll);
     }
     if (my >= 0)
-      aux[my] = outv;
+      aux[my] = emitted(r_o, d_o) | (emitted(r_m, d_m) << 9) | (emitted(r_l, d_l) << 18);
   }
   __threadfence_block();
   __syncthreads();
