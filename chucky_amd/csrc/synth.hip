@@ -13,6 +13,7 @@
 namespace chucky {
 void set_error(const std::string& msg);
 hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
+void host_release(void* p);
 }
 
 namespace {
@@ -178,8 +179,7 @@ done:
     (void)hipStreamDestroy(st);
   if (d)
     (void)hipFree(d);
-  if (h)
-    (void)hipHostFree(h);
+  chucky::host_release(h); // back to the pinned pool it may have come from
   if (gbps < 0)
     chucky::set_error("chucky_measure_link_gbps failed");
   return gbps;
