@@ -52,22 +52,69 @@ __device__ __forceinline__ void plan_body(
   for (uint64_t i = t; i < n_slots; i += kPlanThreads)
     chunk_sizes[i] = 0;
   __syncthreads();
-  for (uint64_t i0 = t; i0 < n_items; i0 += (uint64_t)kPlanThreads * kBatch) {
-    uint32_t g[kBatch], pm[kBatch];
+  if (src.seg_sizes && src.nseg > 1) {
+    // Items of several segments: G lanes (a power of two, up to the whole
+    // wavefront) sum one item's segment sizes side by side, so an item costs
+    // one round of loads instead of one load latency per segment; each
+    // thread holds kBatch items' loads in flight.
+    const uint32_t nseg = src.nseg;
+    uint32_t G = 1;
+    while (G < nseg && G < 64)
+      G <<= 1;
+    const uint32_t lane = (uint32_t)t & 63u, gl = lane & (G - 1);
+    const uint64_t per_wave = 64 / G;
+    const uint64_t round = (uint64_t)(kPlanThreads / 64) * per_wave;
+    const uint64_t mine = (uint64_t)(t >> 6) * per_wave + lane / G;
+    for (uint64_t r0 = 0; r0 < n_items; r0 += round * kBatch) {
+      uint32_t g[kBatch], pm[kBatch];
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q) {
-      const uint64_t i = i0 + (uint64_t)q * kPlanThreads;
-      g[q] = i < n_items ? gather[i] : 0u;
-      pm[q] = i < n_items ? perm[i] : 0u;
+      for (int q = 0; q < kBatch; ++q) {
+        const uint64_t i = r0 + (uint64_t)q * round + mine;
+        g[q] = i < n_items ? gather[i] : 0u;
+        pm[q] = i < n_items ? perm[i] : 0u;
+      }
+      uint64_t v[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const uint64_t i = r0 + (uint64_t)q * round + mine;
+        const uint32_t* __restrict__ ss = src.seg_sizes + (uint64_t)g[q] * nseg;
+        uint64_t acc = 0;
+        if (i < n_items)
+          for (uint32_t k = gl; k < nseg; k += G)
+            acc += ss[k];
+        v[q] = acc;
+      }
+      // sum within each group of G lanes (every lane takes part)
+      for (uint32_t off = G >> 1; off; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+          const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v[q], (int)off, 64);
+          const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v[q] >> 32), (int)off, 64);
+          v[q] += ((uint64_t)hi << 32) | lo;
+        }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q)
+        if (gl == 0 && r0 + (uint64_t)q * round + mine < n_items)
+          chunk_sizes[pm[q]] = v[q];
     }
-    uint64_t v[kBatch];
+  } else {
+    for (uint64_t i0 = t; i0 < n_items; i0 += (uint64_t)kPlanThreads * kBatch) {
+      uint32_t g[kBatch], pm[kBatch];
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q)
-      v[q] = item_bytes(src, g[q]);
+      for (int q = 0; q < kBatch; ++q) {
+        const uint64_t i = i0 + (uint64_t)q * kPlanThreads;
+        g[q] = i < n_items ? gather[i] : 0u;
+        pm[q] = i < n_items ? perm[i] : 0u;
+      }
+      uint64_t v[kBatch];
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q)
-      if (i0 + (uint64_t)q * kPlanThreads < n_items)
-        chunk_sizes[pm[q]] = v[q];
+      for (int q = 0; q < kBatch; ++q)
+        v[q] = item_bytes(src, g[q]);
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q)
+        if (i0 + (uint64_t)q * kPlanThreads < n_items)
+          chunk_sizes[pm[q]] = v[q];
+    }
   }
   __syncthreads();
 
