@@ -142,18 +142,22 @@ hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t 
                            uint32_t* seg_sizes, int level, uint32_t block, void* scratch, uint32_t workers,
                            uint32_t* queue_counter, void* tables, hipStream_t stream);
 size_t zstd_predefined_bytes();
-// launch_zstd over segments of seg_block bytes, each compressed as two blocks
-// of seg_block / 2 by two wavefronts (twice the parallelism for launches of
-// few chunks, e.g. coarse LOD levels), the second block then moved behind
-// the first: the same segment layout and seg_sizes as launch_zstd with block
-// seg_block. sub_sizes: 2 u32 per segment of scratch. Needs
-// zstd_halves_ok(chunk_bytes, seg_block), a queue counter and 16-byte
-// aligned chunks.
-bool zstd_halves_ok(uint64_t chunk_bytes, uint32_t seg_block);
-hipError_t launch_zstd_halves(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
-                              uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
-                              uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
-                              uint32_t* queue_counter, hipStream_t stream);
+// launch_zstd over segments of seg_block bytes, each compressed as `parts`
+// blocks of seg_block / parts by as many wavefronts (more parallelism for
+// launches of few chunks, e.g. coarse LOD levels: a launch lasts as long as
+// its slowest block), the later blocks then moved behind the first: the same
+// segment layout and seg_sizes as launch_zstd with block seg_block. parts =
+// zstd_parts(chunk_bytes, seg_block): the most blocks per segment (up to
+// kZstdMaxParts) of at least kZstdPartMin bytes that the worker stages in LDS,
+// 0 when the segments cannot be split. sub_sizes: kZstdMaxParts u32 per
+// segment of scratch. Needs a queue counter and 16-byte aligned chunks.
+constexpr uint32_t kZstdPartMin = 2 * 1024;
+constexpr uint32_t kZstdMaxParts = 4;
+uint32_t zstd_parts(uint64_t chunk_bytes, uint32_t seg_block);
+hipError_t launch_zstd_parts(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                             uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
+                             uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
+                             uint32_t* queue_counter, hipStream_t stream);
 // Per-chunk frame sizes into sizes[0, n) from the segment sizes of a batch
 // compress; with nseg > 1 (zstd blocks at seg_stride) segments 1.. of each
 // chunk are first moved down behind segment 0, so every frame is contiguous

@@ -68,8 +68,8 @@ struct BatchedState
   bool graphs = false;
   std::vector<hipStream_t> st_zip;
   uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
-  // zstd split epochs: per compression queue, the sizes of the half-segment
-  // blocks the coarse levels are compressed in (2 u32 per segment)
+  // zstd split epochs: per compression queue, the sizes of the part-segment
+  // blocks the coarse levels are compressed in (kZstdMaxParts u32 per segment)
   uint32_t* d_sub_sizes = nullptr;
   uint64_t sub_stride = 0;
   // LZ4 segment scratch, one slice of lz4_stride bytes per compression queue
@@ -259,7 +259,7 @@ int launch_zstd_coarse(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c
   if (!b->d_sub_sizes || c1 <= c0)
     return launch_zstd_range(s, slot, z, c0, c1, st);
   const uint64_t T = s->plan.total_chunks;
-  BCHECK(launch_zstd_halves(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes + c0 * s->chunk_stride_bytes,
+  BCHECK(launch_zstd_parts(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes + c0 * s->chunk_stride_bytes,
                             s->chunk_stride_bytes, s->chunk_bytes, c1 - c0,
                             s->d_comp + ((uint64_t)slot * T + c0) * s->chunk_slot, s->chunk_slot,
                             s->d_seg_sizes + ((uint64_t)slot * T + c0) * s->nseg, b->d_sub_sizes + z * b->sub_stride,
@@ -575,10 +575,10 @@ int batched_init(tile_stream_gpu* s)
   if (dev_alloc(s, &b->d_queues, 16 * b->NZ))
     return 1;
   // split epochs (see batched_submit) compress their coarse levels in
-  // half-segment blocks
+  // blocks of a part of a segment
   if (b->NZ > 1 && b->dma && !b->graphs && p.nlod > 1 && p.nlod <= kAggMaxLevels &&
-      p.config.codec.id == CODEC_ZSTD && zstd_halves_ok(s->chunk_bytes, s->zblock)) {
-    b->sub_stride = align_up((uint64_t)p.total_chunks * s->nseg * 2, 64);
+      p.config.codec.id == CODEC_ZSTD && zstd_parts(s->chunk_bytes, s->zblock)) {
+    b->sub_stride = align_up((uint64_t)p.total_chunks * s->nseg * kZstdMaxParts, 64);
     if (dev_alloc(s, &b->d_sub_sizes, b->NZ * b->sub_stride))
       return 1;
   }

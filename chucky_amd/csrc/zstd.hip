@@ -1614,8 +1614,8 @@ template<> struct WorkerLds<true>
 
 // Work item = one block of `block` bytes. Output segments (one per `sub`
 // consecutive blocks) sit seg_pitch apart in a chunk's slot; block h of a
-// segment writes at h * sub_off in it (sub = 2: two half-segment blocks by
-// two wavefronts, joined afterwards by zstd_join_k). sizes[chunk * nseg * sub
+// segment writes at h * sub_off in it (sub > 1: the blocks of a segment by
+// as many wavefronts, joined afterwards by zstd_join_k). sizes[chunk * nseg * sub
 // + block index] receives each block's bytes (frame header included).
 template<bool Staged>
 __global__ __launch_bounds__(64, 4) void zstd_k(const uint8_t* __restrict__ src, uint64_t src_stride,
@@ -1825,14 +1825,15 @@ hipError_t launch_zstd_pre(const void* predefined, const uint8_t* src, uint64_t 
 }
 
 namespace {
-// One wavefront per segment of a two-block-per-segment launch: the second
-// block's bytes move down behind the first's (the ranges may overlap; the
-// destination lies below the source and every round reads its 1 KiB before
-// writing it, so no byte is overwritten before it is read) and the segment's
-// size becomes their sum.
+// One wavefront per segment of a several-blocks-per-segment launch: blocks
+// 1 .. parts-1 move down, in order, behind the blocks before them (a block's
+// source lies at or above its destination, each round reads its 1 KiB before
+// writing it, and a block's destination ends below the next block's source,
+// so no byte is overwritten before it is read) and the segment's size becomes
+// their sum.
 __global__ __launch_bounds__(64) void zstd_join_k(uint8_t* __restrict__ dst, uint64_t chunk_dst_stride,
-                                                  uint32_t nseg, uint64_t seg_pitch, uint32_t sub_off,
-                                                  const uint32_t* __restrict__ sub_sizes,
+                                                  uint32_t nseg, uint32_t parts, uint64_t seg_pitch,
+                                                  uint32_t sub_off, const uint32_t* __restrict__ sub_sizes,
                                                   uint32_t* __restrict__ seg_sizes, uint64_t n_segments)
 {
   const uint64_t g = blockIdx.x;
@@ -1840,62 +1841,70 @@ __global__ __launch_bounds__(64) void zstd_join_k(uint8_t* __restrict__ dst, uin
     return;
   const uint64_t chunk = g / nseg;
   const uint32_t b = (uint32_t)(g % nseg);
-  const uint32_t a_sz = sub_sizes[2 * g], b_sz = sub_sizes[2 * g + 1];
   uint8_t* seg = dst + chunk * chunk_dst_stride + (uint64_t)b * seg_pitch;
-  const uint8_t* from = seg + sub_off;
-  uint8_t* to = seg + a_sz;
   const int lane = lane_id();
-  for (uint32_t o = 0; o < b_sz; o += 16u * kWave) {
-    uint8_t v[16];
-    const uint32_t q = o + 16u * (uint32_t)lane;
-    const uint32_t k = q < b_sz ? min(16u, b_sz - q) : 0u;
-    for (uint32_t i = 0; i < k; ++i)
-      v[i] = from[q + i];
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = 0; i < k; ++i)
-      to[q + i] = v[i];
-    __builtin_amdgcn_wave_barrier();
+  uint32_t at = sub_sizes[(uint64_t)parts * g];
+  for (uint32_t j = 1; j < parts; ++j) {
+    const uint32_t n = sub_sizes[(uint64_t)parts * g + j];
+    const uint8_t* from = seg + (uint64_t)j * sub_off;
+    uint8_t* to = seg + at;
+    for (uint32_t o = 0; o < n; o += 16u * kWave) {
+      uint8_t v[16];
+      const uint32_t q = o + 16u * (uint32_t)lane;
+      const uint32_t k = q < n ? min(16u, n - q) : 0u;
+      for (uint32_t i = 0; i < k; ++i)
+        v[i] = from[q + i];
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t i = 0; i < k; ++i)
+        to[q + i] = v[i];
+      __builtin_amdgcn_wave_barrier();
+    }
+    at += n;
   }
   if (lane == 0)
-    seg_sizes[g] = a_sz + b_sz;
+    seg_sizes[g] = at;
 }
 } // namespace
 
-bool zstd_halves_ok(uint64_t chunk_bytes, uint32_t seg_block)
+uint32_t zstd_parts(uint64_t chunk_bytes, uint32_t seg_block)
 {
-  return seg_block >= 2 * kZstdBlockMin && seg_block <= 64 * 1024 && chunk_bytes % seg_block == 0 &&
-         seg_block / 2 <= kZstdStageBytes;
+  if (seg_block > 64 * 1024 || chunk_bytes % seg_block)
+    return 0;
+  for (uint32_t parts : { kZstdMaxParts, 2u })
+    if (seg_block / parts >= kZstdPartMin && seg_block / parts <= kZstdStageBytes)
+      return parts;
+  return 0;
 }
 
-hipError_t launch_zstd_halves(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
-                              uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
-                              uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
-                              uint32_t* queue_counter, hipStream_t stream)
+hipError_t launch_zstd_parts(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
+                             uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
+                             uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
+                             uint32_t* queue_counter, hipStream_t stream)
 {
   if (n == 0)
     return hipSuccess;
   const zcore::Predefined* pre = predefined_on_current_device();
   if (!pre)
     return hipErrorNotInitialized;
-  if (!zstd_halves_ok(chunk_bytes, seg_block) || !queue_counter || (src_stride % 16) ||
-      ((uintptr_t)src % 16))
+  const uint32_t parts = zstd_parts(chunk_bytes, seg_block);
+  if (!parts || !queue_counter || (src_stride % 16) || ((uintptr_t)src % 16))
     return hipErrorInvalidValue;
-  const uint32_t block = seg_block / 2;
+  const uint32_t block = seg_block / parts;
   const uint32_t nseg = zstd_segments(chunk_bytes, seg_block);
   const uint64_t pitch = zstd_segment_stride(seg_block);
-  // the first block of a segment writes at most its frame header, block
-  // header and raw bytes below sub_off; the second fits above it
-  const uint32_t sub_off = (uint32_t)(pitch / 2) & ~15u;
-  const uint64_t total = n * nseg * 2;
+  // a block writes at most its frame header, block header and raw bytes:
+  // block j of a segment starts at j * sub_off, which leaves that much room
+  const uint32_t sub_off = (uint32_t)(pitch / parts) & ~15u;
+  const uint64_t total = n * nseg * parts;
   hipLaunchKernelGGL(zstd_k<true>, dim3(workers), dim3(64), 0, stream, src, src_stride, (uint32_t)chunk_bytes,
-                     block, nseg, 2u, sub_off, pitch, total, dst, chunk_dst_stride, sub_sizes,
+                     block, nseg, parts, sub_off, pitch, total, dst, chunk_dst_stride, sub_sizes,
                      (uint8_t*)scratch, queue_counter, pre, (const ChunkTable*)nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
   const uint64_t segs = n * nseg;
-  hipLaunchKernelGGL(zstd_join_k, dim3((uint32_t)segs), dim3(64), 0, stream, dst, chunk_dst_stride, nseg, pitch,
-                     sub_off, sub_sizes, seg_sizes, segs);
+  hipLaunchKernelGGL(zstd_join_k, dim3((uint32_t)segs), dim3(64), 0, stream, dst, chunk_dst_stride, nseg, parts,
+                     pitch, sub_off, sub_sizes, seg_sizes, segs);
   return hipGetLastError();
 }
 
