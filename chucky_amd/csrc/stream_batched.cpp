@@ -67,10 +67,6 @@ struct BatchedState
   // per-epoch compress and aggregate as captured graphs (else direct launches)
   bool graphs = false;
   std::vector<hipStream_t> st_zip;
-  // per compression queue: recorded after each compress launch on it, so a
-  // launch that borrows the queue's scratch and work counter on the compute
-  // queue follows the queue's last one
-  hipEvent_t zdone[kMaxZip] = {};
   uint32_t* d_queues = nullptr; // NZ work counters, 64 bytes apart
   // zstd split epochs: per compression queue, the sizes of the part-segment
   // blocks the coarse levels are compressed in (kZstdMaxParts u32 per segment)
@@ -570,8 +566,6 @@ int batched_init(tile_stream_gpu* s)
     b->st_zip.assign(b->NZ, nullptr);
     for (auto& z : b->st_zip)
       BCHECK(hipStreamCreateWithFlags(&z, hipStreamNonBlocking));
-    for (uint32_t z = 0; z < b->NZ; ++z)
-      BCHECK(hipEventCreateWithFlags(&b->zdone[z], hipEventDisableTiming));
   } else {
     BCHECK(hipStreamCreateWithFlags(&b->st_agg, hipStreamNonBlocking));
   }
@@ -701,9 +695,6 @@ void batched_free(tile_stream_gpu* s)
     (void)hipEventDestroy(e);
   for (auto e : b->timing_pool)
     (void)hipEventDestroy(e);
-  for (auto e : b->zdone)
-    if (e)
-      (void)hipEventDestroy(e);
   if (b->st_agg)
     (void)hipStreamDestroy(b->st_agg);
   for (auto c : b->st_copy)
@@ -718,7 +709,7 @@ void batched_free(tile_stream_gpu* s)
 
 // Epoch `pool_slot` of the pool is complete (scatter / LOD already queued on
 // the compute queue): compress, aggregate and queue it for delivery.
-int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool last)
+int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
 {
   BatchedState* b = s->batched;
   const uint64_t ticket = b->next_epoch++;
@@ -734,33 +725,19 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool l
   auto& P = b->pool[pool_slot];
   s->slot_compressed[pool_slot] = 1;
   const Plan& p = s->plan;
-  // The epoch that ends an append call (or a flush) has no successor whose
-  // LOD or scatter would queue behind its level-0 compression on the compute
-  // queue. When that compression is one short launch (no more blocks than
-  // compressor slots) it runs there, right behind the LOD, instead of behind
-  // a wait on another queue (tens of microseconds from the LOD's end to the
-  // compression's start); it borrows its compression queue's scratch and work
-  // counter after that queue's last launch.
-  const uint64_t n_level0 = p.nlod > 1 ? p.levels[1].chunk_offset : p.total_chunks;
-  const bool inline_l0 = last && b->NZ > 1 && b->dma && !b->graphs && p.config.codec.id == CODEC_ZSTD &&
-                         !s->d_ztables && (mask & 1u) && n_level0 * s->nseg <= b->zip_workers;
   // An epoch that also emits coarse levels is split over both compression
   // queues: the coarse levels' chunks (their blocks are the slow ones) start
   // first on the slot's queue, level 0 beside them on the other queue, and
   // level 0's gather and copy leave while the coarse levels still compress.
   if (b->NZ > 1 && b->dma && !b->graphs && p.nlod > 1 && p.nlod <= kAggMaxLevels &&
       p.config.codec.id == CODEC_ZSTD && (mask & 1u) && (mask & ~1u)) {
-    const uint32_t zib = pool_slot % b->NZ, zia = (pool_slot + 1) % b->NZ;
-    hipStream_t zb = b->st_zip[zib], za = inline_l0 ? s->st_comp : b->st_zip[zia];
+    hipStream_t zb = b->st_zip[pool_slot % b->NZ], za = b->st_zip[(pool_slot + 1) % b->NZ];
     const uint64_t n0 = p.levels[1].chunk_offset; // level 0 chunks come first
     BCHECK(hipEventRecord(P.ev_filled, s->st_comp));
     for (hipStream_t z : { zb, za }) {
-      if (z != s->st_comp)
-        BCHECK(hipStreamWaitEvent(z, P.ev_filled, 0));
+      BCHECK(hipStreamWaitEvent(z, P.ev_filled, 0));
       BCHECK(hipStreamWaitEvent(z, P.ev_gathered, 0));
     }
-    if (inline_l0)
-      BCHECK(hipStreamWaitEvent(za, b->zdone[zia], 0));
     const uint64_t n1 = active_chunk_end(p, mask);
     hipEvent_t c0 = take_timing_event(b), c1 = take_timing_event(b);
     hipEvent_t c2 = take_timing_event(b), c3 = take_timing_event(b);
@@ -770,16 +747,13 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool l
       return 1;
     }
     BCHECK(hipEventRecord(c0, zb));
-    if (launch_zstd_coarse(s, pool_slot, zib, n0, n1, zb))
+    if (launch_zstd_coarse(s, pool_slot, pool_slot % b->NZ, n0, n1, zb))
       return 1;
     BCHECK(hipEventRecord(c1, zb));
-    BCHECK(hipEventRecord(b->zdone[zib], zb));
     BCHECK(hipEventRecord(c2, za));
-    if (launch_zstd_range(s, pool_slot, zia, 0, n0, za))
+    if (launch_zstd_range(s, pool_slot, (pool_slot + 1) % b->NZ, 0, n0, za))
       return 1;
     BCHECK(hipEventRecord(c3, za));
-    if (!inline_l0)
-      BCHECK(hipEventRecord(b->zdone[zia], za));
     BCHECK(hipEventRecord(P.ev_compressed, za));
     if (launch_aggregate(s, b, pool_slot, hs, 1u, za, 0))
       return 1;
@@ -815,9 +789,7 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool l
   // the slot's compression queue takes over once the epoch is in the pool;
   // compressed slots of this pool slot are free once its last gather is done
   hipStream_t zs = s->st_comp;
-  if (inline_l0) {
-    BCHECK(hipStreamWaitEvent(zs, b->zdone[pool_slot % b->NZ], 0));
-  } else if (b->NZ > 1) {
+  if (b->NZ > 1) {
     zs = b->st_zip[pool_slot % b->NZ];
     BCHECK(hipEventRecord(P.ev_filled, s->st_comp));
     BCHECK(hipStreamWaitEvent(zs, P.ev_filled, 0));
@@ -841,8 +813,6 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool l
     else if (launch_compress(s, pool_slot, zs, n_comp))
       return 1;
     BCHECK(hipEventRecord(c1, zs));
-    if (b->NZ > 1 && !inline_l0)
-      BCHECK(hipEventRecord(b->zdone[pool_slot % b->NZ], zs));
   }
   BCHECK(hipEventRecord(P.ev_compressed, zs));
   // the aggregate runs on the compression queue after the compress, or on its

@@ -406,16 +406,15 @@ int linear_prepare(tile_stream_gpu* s)
   return 0;
 }
 
-int end_epoch(tile_stream_gpu* s, bool last);
+int end_epoch(tile_stream_gpu* s);
 
-// Queues the launches of an epoch whose end was deferred (epoch_pending);
-// `last`: no further epoch follows in this append call.
-int settle_epoch(tile_stream_gpu* s, bool last)
+// Queues the launches of an epoch whose end was deferred (epoch_pending).
+int settle_epoch(tile_stream_gpu* s)
 {
   if (!s->epoch_pending)
     return 0;
   s->epoch_pending = false;
-  return end_epoch(s, last);
+  return end_epoch(s);
 }
 
 // The staged piece, in two steps. dispatch_h2d: the H2D copy into the
@@ -447,7 +446,7 @@ int dispatch_h2d(tile_stream_gpu* s)
 int dispatch_rest(tile_stream_gpu* s)
 {
   const size_t bytes = s->stage_fill;
-  if (settle_epoch(s, false))
+  if (settle_epoch(s))
     return 1;
   if (bytes == 0)
     return 0;
@@ -523,8 +522,7 @@ int update_metadata(tile_stream_gpu* s, bool final)
 }
 
 // Epoch boundary: LOD (multiscale), record the active mask, flush at K.
-// `last`: the epoch ends the append call (nothing of it is queued after).
-int end_epoch(tile_stream_gpu* s, bool last)
+int end_epoch(tile_stream_gpu* s)
 {
   uint32_t mask = 1;
   if (s->plan.enable_multiscale) {
@@ -533,7 +531,7 @@ int end_epoch(tile_stream_gpu* s, bool last)
   }
   if (s->batched) {
     s->epochs_done++;
-    if (batched_submit(s, s->batch_acc, mask, last))
+    if (batched_submit(s, s->batch_acc, mask))
       return 1;
     s->batch_acc = (s->batch_acc + 1) % s->plan.epochs_per_batch;
     if (batched_acquire_slot(s, s->batch_acc))
@@ -575,7 +573,7 @@ int flush_all(tile_stream_gpu* s)
       return 1;
     if (s->batched) {
       s->epochs_done++;
-      if (batched_submit(s, s->batch_acc, mask, true))
+      if (batched_submit(s, s->batch_acc, mask))
         return 1;
       s->batch_acc = (s->batch_acc + 1) % s->plan.epochs_per_batch;
     } else {
@@ -771,14 +769,14 @@ writer_result append_impl(tile_stream_gpu* s, const uint8_t* src, const uint8_t*
       const bool more = !DEVICE && (uint64_t)(end - src) >= bpe && !(maxc && s->cursor >= maxc);
       if (more) {
         s->epoch_pending = true;
-      } else if (end_epoch(s, true)) {
+      } else if (end_epoch(s)) {
         prefetch_drop(s);
         return writer_error_at(src, end);
       }
     }
   }
   prefetch_drop(s);
-  if (settle_epoch(s, true))
+  if (settle_epoch(s))
     return writer_error_at(src, end);
   return writer_result{ 0, { src, end } };
 }

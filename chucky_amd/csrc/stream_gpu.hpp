@@ -198,7 +198,7 @@ uint32_t tune_h2d_copy_workgroups();
 uint64_t lz4_wave_slots(int hash_log, uint32_t seg_bytes);
 int batched_init(tile_stream_gpu* s);
 void batched_free(tile_stream_gpu* s);
-int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask, bool last);
+int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask);
 int batched_acquire_slot(tile_stream_gpu* s, uint32_t pool_slot);
 int batched_drain(tile_stream_gpu* s);
 void batched_reset(tile_stream_gpu* s);
