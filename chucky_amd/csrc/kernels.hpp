@@ -151,13 +151,15 @@ size_t zstd_predefined_bytes();
 // kZstdMaxParts) of at least kZstdPartMin bytes that the worker stages in LDS,
 // 0 when the segments cannot be split. sub_sizes: kZstdMaxParts u32 per
 // segment of scratch. Needs a queue counter and 16-byte aligned chunks.
+// `between` (optional) is recorded on the stream after the compression and
+// before the join.
 constexpr uint32_t kZstdPartMin = 2 * 1024;
 constexpr uint32_t kZstdMaxParts = 4;
 uint32_t zstd_parts(uint64_t chunk_bytes, uint32_t seg_block);
 hipError_t launch_zstd_parts(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                              uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
                              uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
-                             uint32_t* queue_counter, hipStream_t stream);
+                             uint32_t* queue_counter, hipStream_t stream, hipEvent_t between = nullptr);
 // Per-chunk frame sizes into sizes[0, n) from the segment sizes of a batch
 // compress; with nseg > 1 (zstd blocks at seg_stride) segments 1.. of each
 // chunk are first moved down behind segment 0, so every frame is contiguous

@@ -253,18 +253,24 @@ int launch_zstd_range(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0
 // zstd on the coarse chunks [c0, c1) of a split epoch: up to four wavefronts
 // per block-sized segment when the layout allows (these launches hold few
 // chunks, and coarse blocks are the slowest), else as launch_zstd_range.
-int launch_zstd_coarse(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0, uint64_t c1, hipStream_t st)
+// `done` is recorded once the compression kernel has run (before the join).
+int launch_zstd_coarse(tile_stream_gpu* s, uint32_t slot, uint32_t z, uint64_t c0, uint64_t c1, hipStream_t st,
+                       hipEvent_t done)
 {
   const BatchedState* b = s->batched;
-  if (!b->d_sub_sizes || c1 <= c0)
-    return launch_zstd_range(s, slot, z, c0, c1, st);
+  if (!b->d_sub_sizes || c1 <= c0) {
+    if (launch_zstd_range(s, slot, z, c0, c1, st))
+      return 1;
+    BCHECK(hipEventRecord(done, st));
+    return 0;
+  }
   const uint64_t T = s->plan.total_chunks;
   BCHECK(launch_zstd_parts(s->d_pool + (uint64_t)slot * s->epoch_pool_bytes + c0 * s->chunk_stride_bytes,
                             s->chunk_stride_bytes, s->chunk_bytes, c1 - c0,
                             s->d_comp + ((uint64_t)slot * T + c0) * s->chunk_slot, s->chunk_slot,
                             s->d_seg_sizes + ((uint64_t)slot * T + c0) * s->nseg, b->d_sub_sizes + z * b->sub_stride,
                             s->zblock, (uint8_t*)s->d_zscratch + zstd_scratch_bytes(z * b->zip_workers),
-                            b->zip_workers, b->d_queues + 16 * z, st));
+                            b->zip_workers, b->d_queues + 16 * z, st, done));
   return 0;
 }
 
@@ -747,9 +753,8 @@ int batched_submit(tile_stream_gpu* s, uint32_t pool_slot, uint32_t mask)
       return 1;
     }
     BCHECK(hipEventRecord(c0, zb));
-    if (launch_zstd_coarse(s, pool_slot, pool_slot % b->NZ, n0, n1, zb))
+    if (launch_zstd_coarse(s, pool_slot, pool_slot % b->NZ, n0, n1, zb, c1))
       return 1;
-    BCHECK(hipEventRecord(c1, zb));
     BCHECK(hipEventRecord(c2, za));
     if (launch_zstd_range(s, pool_slot, (pool_slot + 1) % b->NZ, 0, n0, za))
       return 1;

@@ -1879,10 +1879,10 @@ uint32_t zstd_parts(uint64_t chunk_bytes, uint32_t seg_block)
 hipError_t launch_zstd_parts(const uint8_t* src, uint64_t src_stride, uint64_t chunk_bytes, uint64_t n,
                              uint8_t* dst, uint64_t chunk_dst_stride, uint32_t* seg_sizes,
                              uint32_t* sub_sizes, uint32_t seg_block, void* scratch, uint32_t workers,
-                             uint32_t* queue_counter, hipStream_t stream)
+                             uint32_t* queue_counter, hipStream_t stream, hipEvent_t between)
 {
   if (n == 0)
-    return hipSuccess;
+    return between ? hipEventRecord(between, stream) : hipSuccess;
   const zcore::Predefined* pre = predefined_on_current_device();
   if (!pre)
     return hipErrorNotInitialized;
@@ -1901,6 +1901,8 @@ hipError_t launch_zstd_parts(const uint8_t* src, uint64_t src_stride, uint64_t c
                      (uint8_t*)scratch, queue_counter, pre, (const ChunkTable*)nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
+    return e;
+  if (between && (e = hipEventRecord(between, stream)) != hipSuccess)
     return e;
   const uint64_t segs = n * nseg;
   hipLaunchKernelGGL(zstd_join_k, dim3((uint32_t)segs), dim3(64), 0, stream, dst, chunk_dst_stride, nseg, parts,
