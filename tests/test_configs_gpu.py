@@ -80,10 +80,20 @@ def test_multiscale_zstd_256_host_untimed(volumes, pipeline, pieces):
     # the benchmark's form: host memory appends (the first piece of a call
     # staged and copied in sub-ranges, the next ones copied ahead), no event
     # markers between the scatter / LOD launches; the batched pipeline
-    # compresses the coarse levels of the epochs that emit them in half blocks
+    # compresses the coarse levels of the epochs that emit them in four 2 KiB
+    # blocks per 8 KiB segment
     cfg = W.multiscale_zstd()
     _check(cfg, volumes[0], pipeline, pieces=pieces, device_input=False, pass_timing=2,
            what="multiscale_zstd/host")
+
+
+@pytest.mark.parametrize("block", [4096, 16384])
+def test_multiscale_zstd_256_coarse_parts(volumes, block):
+    # the coarse levels of split epochs at other segment sizes: 4 KiB segments
+    # in two 2 KiB blocks, 16 KiB segments in four 4 KiB blocks
+    cfg = W.multiscale_zstd()
+    _check(cfg, volumes[1], "batched", pieces=1, device_input=False, pass_timing=2,
+           zstd_block_bytes=block, what=f"multiscale_zstd/parts{block}")
 
 
 @pytest.mark.parametrize("pipeline", sorted(PIPELINES))
