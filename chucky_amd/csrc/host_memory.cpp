@@ -114,6 +114,8 @@ hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device)
   return e;
 }
 
+hipError_t host_alloc_unpooled(void** p, size_t bytes, int device) { return pin_new(p, bytes, device); }
+
 void host_release(void* p)
 {
   if (!p)

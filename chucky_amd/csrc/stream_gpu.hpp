@@ -180,6 +180,9 @@ hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
 // Returns a buffer of host_alloc_near_gpu to the process's pinned pool (or
 // frees it when the pool is full).
 void host_release(void* p);
+// A new pinned buffer on the GPU's node that bypasses the pool (freed with
+// hipHostFree): for measurements of the host link itself.
+hipError_t host_alloc_unpooled(void** p, size_t bytes, int device);
 // Codec launch shapes from the recorded sweep tables (autotune.cpp), by
 // multiscale or not, dtype size, chunk bytes and chunks per compress launch.
 uint32_t tune_zstd_block(uint64_t chunk_bytes, uint64_t chunks_per_launch, uint32_t bpe, bool lod);

@@ -12,8 +12,7 @@
 
 namespace chucky {
 void set_error(const std::string& msg);
-hipError_t host_alloc_near_gpu(void** p, size_t bytes, int device);
-void host_release(void* p);
+hipError_t host_alloc_unpooled(void** p, size_t bytes, int device);
 }
 
 namespace {
@@ -153,7 +152,7 @@ double chucky_measure_link_gbps(size_t bytes, int reps, int to_device)
   double gbps = -1;
   float ms = 0;
   int dev = 0;
-  if (reps < 1 || hipGetDevice(&dev) != hipSuccess || chucky::host_alloc_near_gpu(&h, bytes, dev) != hipSuccess || hipMalloc(&d, bytes) != hipSuccess ||
+  if (reps < 1 || hipGetDevice(&dev) != hipSuccess || chucky::host_alloc_unpooled(&h, bytes, dev) != hipSuccess || hipMalloc(&d, bytes) != hipSuccess ||
       hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
       hipEventCreate(&b) != hipSuccess)
     goto done;
@@ -179,7 +178,8 @@ done:
     (void)hipStreamDestroy(st);
   if (d)
     (void)hipFree(d);
-  chucky::host_release(h); // back to the pinned pool it may have come from
+  if (h)
+    (void)hipHostFree(h);
   if (gbps < 0)
     chucky::set_error("chucky_measure_link_gbps failed");
   return gbps;
