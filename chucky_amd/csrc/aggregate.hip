@@ -171,12 +171,25 @@ __device__ __forceinline__ void plan_body(
     // slots: a thread's own contiguous range would write pinned memory at a
     // 8 * per byte stride across the wavefront, one PCIe transaction per
     // 8-byte value
+    // (kBatch slots' loads are issued before their stores, so a thread waits
+    // one load latency per kBatch slots, not per slot)
     __syncthreads();
-    for (uint64_t i = t; i <= n_slots; i += kPlanThreads) {
-      if (host_offsets)
-        host_offsets[i] = offsets[i];
-      if (host_sizes && i < n_slots)
-        host_sizes[i] = chunk_sizes[i];
+    for (uint64_t i0 = t; i0 <= n_slots; i0 += (uint64_t)kPlanThreads * kBatch) {
+      uint64_t o[kBatch], z[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const uint64_t i = i0 + (uint64_t)q * kPlanThreads;
+        o[q] = i <= n_slots ? offsets[i] : 0ull;
+        z[q] = i < n_slots ? chunk_sizes[i] : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const uint64_t i = i0 + (uint64_t)q * kPlanThreads;
+        if (host_offsets && i <= n_slots)
+          host_offsets[i] = o[q];
+        if (host_sizes && i < n_slots)
+          host_sizes[i] = z[q];
+      }
     }
   }
 }
